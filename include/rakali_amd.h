@@ -1,0 +1,144 @@
+/*
+ * rakali_amd.h -- C ABI of the MI355X (gfx950) structured 2-D shallow-water
+ * step: the kernel-level boundary ("rkg_*") that replaces the calls
+ * solver_step_impl makes into the structured barotropic kernels.
+ *
+ * Reference interfaces replaced (file:line in JorgeG94/rakali):
+ *   rkg_step / rkg_advance  -> the fused SSP-RK2 branch of solver_step_impl
+ *                              (src/core/coastal/solver/rki_solver.F90:692-815)
+ *                              as driven by rki_step / rki_advance
+ *                              (src/api/rki_api.F90:358-410), i.e. per step:
+ *                              apply_boundary_conditions (rki_boundary.F90:40),
+ *                              compute_timestep (rki_kernel_timestep.F90:16),
+ *                              compute_flux_hll[c] (rki_kernel_flux.F90:17,62),
+ *                              fused_update_stage1/2 (rki_kernel_fused.F90:164,305)
+ *   rkg_create / rkg_destroy -> barotropic_state_init + solver_enter_data /
+ *                              solver_exit_data (rki_barotropic_state.F90:114,
+ *                              rki_solver.F90:131,281)
+ *   rkg_upload / rkg_download-> rki_set_bathymetry / rki_set_initial_h /
+ *                              rki_set_h/hu/hv and the rki_get_*_ptr getters
+ *                              (rki_api.F90:502-686, 836-1123) at the level of
+ *                              device buffers
+ *   rkg_total_mass          -> compute_total_mass (rki_init.F90:287)
+ *
+ * All functions return 0 on success and a negative code on failure, in the
+ * reference's convention (rki_api.F90:22-27): -1 not ready, -2 shape or
+ * argument out of range, -100 invalid handle; plus -4 for a HIP runtime
+ * error and -5 for a configuration this build does not implement.
+ * Arrays on the host side are Fortran-ordered FP64 (x fastest) with a
+ * leading dimension `ld` >= nx, so a caller may pass the interior of a
+ * ghosted (nx_total, ny_total) array as (&a[ng + ng*nx_total], ld=nx_total).
+ */
+#ifndef RAKALI_AMD_H
+#define RAKALI_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RKG_OK 0
+#define RKG_ERR_NOT_READY (-1)
+#define RKG_ERR_RANGE (-2)
+#define RKG_ERR_HIP (-4)
+#define RKG_ERR_UNSUPPORTED (-5)
+#define RKG_ERR_BAD_HANDLE (-100)
+
+/* Boundary tags: rki_boundary_types.F90:18-32. */
+#define RKG_BC_WALL 1
+#define RKG_BC_OPEN 2
+#define RKG_BC_TIDAL 3
+#define RKG_BC_INFLOW 5
+#define RKG_BC_DISCHARGE 6
+#define RKG_BC_CLAMPED 7
+#define RKG_BC_SPONGE 8
+
+/* Field selectors. */
+#define RKG_FIELD_H 0
+#define RKG_FIELD_HU 1
+#define RKG_FIELD_HV 2
+#define RKG_FIELD_B 3
+
+#define RKG_MAX_TIDAL_CONSTITUENTS 10
+
+/* Scalar configuration of one grid; the namelist keys the step reads
+ * (config_t, rki_config.F90:37-260; bc_params_t, rki_boundary_types.F90:37). */
+typedef struct rkg_config {
+    int nx, ny;                 /* physical cells (>= 2 each) */
+    double dx, dy;              /* cell sizes (m) */
+    double g;                   /* gravity; reference GRAVITY = 9.80665 */
+    double cfl;                 /* CFL number (0.45) */
+    double manning_n;           /* scalar Manning n (0 = no friction) */
+    double coriolis_f;          /* Coriolis parameter (1/s) */
+    double dt_fixed;            /* > 0: fixed dt */
+    double t_end;               /* bounds dt in rkg_step (dt_max = t_end - t) */
+    int cfl_interval;           /* dt recomputed every cfl_interval steps */
+    int use_hllc;               /* 1 = HLLC, 0 = HLL (reference default HLL) */
+    int bc[4];                  /* west, east, south, north tags */
+    double tidal_amplitude;     /* single-constituent tide (m) */
+    double tidal_period;        /* (s), default 44712 */
+    int n_tidal_constituents;   /* > 0: constituent sum replaces the above */
+    double tidal_amp[RKG_MAX_TIDAL_CONSTITUENTS];
+    double tidal_phase[RKG_MAX_TIDAL_CONSTITUENTS];
+    double tidal_omega[RKG_MAX_TIDAL_CONSTITUENTS];
+    double inflow_velocity;     /* BC_INFLOW normal velocity (m/s) */
+    double discharge_flux;      /* BC_DISCHARGE normal flux (m^2/s) */
+    double clamped_elevation;   /* BC_CLAMPED surface (m) */
+} rkg_config;
+
+typedef struct rkg_grid rkg_grid;   /* opaque handle */
+
+/* Reference defaults (rki_config.F90, rki_constants.F90). */
+int rkg_config_default(rkg_config *cfg);
+
+/* Allocates device state for one grid on the current HIP device.  `stream`
+ * is a hipStream_t to launch on, or NULL for a library-owned stream. */
+int rkg_create(const rkg_config *cfg, void *stream, rkg_grid **out);
+/* Frees the grid and nulls *g; a NULL or already-destroyed handle is a no-op. */
+void rkg_destroy(rkg_grid **g);
+
+/* Host <-> device copies of one interior field (see the header comment for
+ * `ld`).  Uploading h, hu or hv (or b) marks the CFL state stale; the next
+ * step recomputes it. */
+int rkg_upload(rkg_grid *g, int field, const double *src, int ld);
+int rkg_download(rkg_grid *g, int field, double *dst, int ld);
+/* Device-to-device variants: `src`/`dst` is a device pointer with leading
+ * dimension `ld` elements. */
+int rkg_upload_device(rkg_grid *g, int field, const double *src, int ld);
+int rkg_download_device(rkg_grid *g, int field, double *dst, int ld);
+/* Zero-copy device view of a field: pointer and row pitch (elements). */
+int rkg_field_device_ptr(rkg_grid *g, int field, double **ptr, long long *pitch);
+
+/* Advances n SSP-RK2 steps (rki_step semantics: dt_max = t_end - t).
+ * Asynchronous: returns once the launches are queued. */
+int rkg_step(rkg_grid *g, int n);
+/* Steps while t < target - 1e-10 with dt_max = target - t (rki_advance
+ * semantics).  Synchronous; *steps_taken may be NULL. */
+int rkg_advance(rkg_grid *g, double target, long long *steps_taken);
+
+/* Clock: time, dt of the last step, and step count (synchronizes). */
+int rkg_get_clock(rkg_grid *g, double *t, double *dt, long long *step_count);
+/* Sets the time and step count (e.g. after a restart). */
+int rkg_set_clock(rkg_grid *g, double t, long long step_count);
+/* Total mass, sum of h*dx*dy over interior cells with 0 < h < 1e6, in a
+ * fixed summation order (deterministic). */
+int rkg_total_mass(rkg_grid *g, double *mass);
+/* Waits for all queued work of this grid. */
+int rkg_sync(rkg_grid *g);
+
+/* Rows per tile segment of the stage kernels (1..64; default 32). */
+int rkg_set_tile_rows(rkg_grid *g, int rows);
+int rkg_get_tile_rows(rkg_grid *g, int *rows);
+/* Last-error text of this thread. */
+const char *rkg_last_error(void);
+
+/* Evaluates the step's transcendental functions on n inputs, on the host
+ * (on_device = 0) or the current GPU: fn 0 = x**(1.0/3.0) (Manning,
+ * rki_kernel_fused.F90:268), 1 = sin, 2 = cos (tidal surface
+ * rki_boundary.F90:96-100, Coriolis rki_kernel_fused.F90:212-213).  Both
+ * sides return identical, correctly rounded bits. */
+int rkg_math_eval(int fn, const double *x, double *y, int n, int on_device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,216 @@
+"""ctypes view of the CPU restatement in ``rk_oracle.c``.
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker that the HIP path is
+compared against; only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import it.  The product package
+(``rakali_amd``) never does.
+
+Arrays are numpy float64 in Fortran order with shape ``(nx + 2*ng, ny + 2*ng)``
+-- the reference's ``(nx_total, ny_total)`` column-major layout.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "liborc.so"
+
+BC_WALL, BC_OPEN, BC_TIDAL, BC_NESTED = 1, 2, 3, 4
+BC_INFLOW, BC_DISCHARGE, BC_CLAMPED, BC_SPONGE, BC_CHAPMAN = 5, 6, 7, 8, 9
+BC_NAMES = {"wall": BC_WALL, "open": BC_OPEN, "tidal": BC_TIDAL,
+            "nested": BC_NESTED, "inflow": BC_INFLOW,
+            "discharge": BC_DISCHARGE, "clamped": BC_CLAMPED,
+            "sponge": BC_SPONGE, "chapman": BC_CHAPMAN}
+
+MAX_TC = 10
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("ng", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("g", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("manning_n", ctypes.c_double), ("coriolis_f", ctypes.c_double),
+        ("dt_fixed", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("cfl_interval", ctypes.c_int), ("use_hllc", ctypes.c_int),
+        ("bc", ctypes.c_int * 4),
+        ("tidal_amplitude", ctypes.c_double), ("tidal_period", ctypes.c_double),
+        ("n_tc", ctypes.c_int),
+        ("tc_amp", ctypes.c_double * MAX_TC),
+        ("tc_phase", ctypes.c_double * MAX_TC),
+        ("tc_omega", ctypes.c_double * MAX_TC),
+        ("inflow_velocity", ctypes.c_double),
+        ("discharge_flux", ctypes.c_double),
+        ("clamped_elevation", ctypes.c_double),
+    ]
+
+
+class Clock(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double),
+                ("mass_out", ctypes.c_double),
+                ("step_count", ctypes.c_longlong)]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile liborc.so with the Makefile beside this file."""
+    if force or not _LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(_HERE), "liborc.so"],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(_LIB_PATH))
+        P = ctypes.POINTER(Params)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_default_params.argtypes = [P]
+        L.orc_compute_flux.argtypes = [P] + [dp] * 7 + [ctypes.c_int]
+        L.orc_fused_stage1.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
+        L.orc_fused_stage2.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
+        L.orc_compute_timestep.argtypes = [P, dp, dp, dp]
+        L.orc_compute_timestep.restype = ctypes.c_double
+        L.orc_tidal_eta.argtypes = [P, ctypes.c_double]
+        L.orc_tidal_eta.restype = ctypes.c_double
+        L.orc_apply_bc.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
+        L.orc_fill_bathymetry_ghosts.argtypes = [P, dp]
+        L.orc_init_still_water.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
+        L.orc_total_mass.argtypes = [P, dp]
+        L.orc_total_mass.restype = ctypes.c_double
+        L.orc_run.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(Clock),
+                              ctypes.c_int, ctypes.c_longlong, ctypes.c_double]
+        L.orc_run.restype = ctypes.c_longlong
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["F_CONTIGUOUS"], "need F-ordered float64"
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_params(nx, ny, ng=2, dx=1.0, dy=1.0, **kw) -> Params:
+    p = Params()
+    lib().orc_default_params(ctypes.byref(p))
+    p.nx, p.ny, p.ng, p.dx, p.dy = nx, ny, ng, dx, dy
+    bcs = kw.pop("bc", None)
+    if bcs is not None:
+        for s, v in enumerate(bcs):
+            p.bc[s] = BC_NAMES[v] if isinstance(v, str) else int(v)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def zeros(p: Params) -> np.ndarray:
+    return np.zeros((p.nx + 2 * p.ng, p.ny + 2 * p.ng), dtype=np.float64, order="F")
+
+
+def compute_flux(p, h, hu, hv, b, hllc=None):
+    fh, fhu, fhv = zeros(p), zeros(p), zeros(p)
+    use = p.use_hllc if hllc is None else int(hllc)
+    lib().orc_compute_flux(ctypes.byref(p), _ptr(h), _ptr(hu), _ptr(hv), _ptr(b),
+                           _ptr(fh), _ptr(fhu), _ptr(fhv), use)
+    return fh, fhu, fhv
+
+
+def fused_stage1(p, h, hu, hv, fh, fhu, fhv, dt):
+    h0, hu0, hv0 = zeros(p), zeros(p), zeros(p)
+    lib().orc_fused_stage1(ctypes.byref(p), _ptr(h), _ptr(hu), _ptr(hv),
+                           _ptr(h0), _ptr(hu0), _ptr(hv0),
+                           _ptr(fh), _ptr(fhu), _ptr(fhv), dt)
+    return h0, hu0, hv0
+
+
+def fused_stage2(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt):
+    lib().orc_fused_stage2(ctypes.byref(p), _ptr(h), _ptr(hu), _ptr(hv),
+                           _ptr(h0), _ptr(hu0), _ptr(hv0),
+                           _ptr(fh), _ptr(fhu), _ptr(fhv), dt)
+
+
+def compute_timestep(p, h, hu, hv) -> float:
+    return lib().orc_compute_timestep(ctypes.byref(p), _ptr(h), _ptr(hu), _ptr(hv))
+
+
+def tidal_eta(p, t) -> float:
+    return lib().orc_tidal_eta(ctypes.byref(p), t)
+
+
+def apply_bc(p, time, h, hu, hv, b):
+    lib().orc_apply_bc(ctypes.byref(p), time, _ptr(h), _ptr(hu), _ptr(hv), _ptr(b))
+
+
+def fill_bathymetry_ghosts(p, b):
+    lib().orc_fill_bathymetry_ghosts(ctypes.byref(p), _ptr(b))
+
+
+def total_mass(p, h) -> float:
+    return lib().orc_total_mass(ctypes.byref(p), _ptr(h))
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().orc_get_threads()
+
+
+class Sim:
+    """Handle-like driver of the restated step (two-phase lifecycle:
+    set the interior bed and depth, then ``finalize`` fills bed ghosts)."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        self.h, self.hu, self.hv, self.b = zeros(p), zeros(p), zeros(p), zeros(p)
+        self.clock = Clock()
+
+    @property
+    def ng(self):
+        return self.p.ng
+
+    def interior(self, a):
+        g = self.p.ng
+        return a[g:g + self.p.nx, g:g + self.p.ny]
+
+    def set_interior(self, name, arr):
+        self.interior(getattr(self, name))[...] = np.asarray(arr, dtype=np.float64)
+
+    def finalize(self):
+        fill_bathymetry_ghosts(self.p, self.b)
+
+    def step(self, n: int) -> int:
+        return lib().orc_run(ctypes.byref(self.p), _ptr(self.h), _ptr(self.hu),
+                             _ptr(self.hv), _ptr(self.b), ctypes.byref(self.clock),
+                             0, int(n), 0.0)
+
+    def advance(self, target: float) -> int:
+        return lib().orc_run(ctypes.byref(self.p), _ptr(self.h), _ptr(self.hu),
+                             _ptr(self.hv), _ptr(self.b), ctypes.byref(self.clock),
+                             1, 0, float(target))
+
+    @property
+    def t(self):
+        return self.clock.t
+
+    @property
+    def dt(self):
+        return self.clock.dt
+
+    def mass(self):
+        return total_mass(self.p, self.h)

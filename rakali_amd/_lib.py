@@ -1,0 +1,104 @@
+"""ctypes binding of ``librakali_amd.so`` (the ``rkg_*`` C ABI in
+``include/rakali_amd.h``).
+
+The library is the only compute path: if it is missing or fails to load,
+every call raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "librakali_amd.so"
+
+MAX_TC = 10
+
+RKG_OK = 0
+ERRORS = {-1: "not ready", -2: "argument out of range", -4: "HIP runtime error",
+          -5: "unsupported configuration", -100: "invalid handle"}
+
+BC = {"wall": 1, "open": 2, "tidal": 3, "inflow": 5, "discharge": 6,
+      "clamped": 7, "sponge": 8}
+FIELDS = {"h": 0, "hu": 1, "hv": 2, "b": 3}
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("g", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("manning_n", ctypes.c_double), ("coriolis_f", ctypes.c_double),
+        ("dt_fixed", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("cfl_interval", ctypes.c_int), ("use_hllc", ctypes.c_int),
+        ("bc", ctypes.c_int * 4),
+        ("tidal_amplitude", ctypes.c_double), ("tidal_period", ctypes.c_double),
+        ("n_tidal_constituents", ctypes.c_int),
+        ("tidal_amp", ctypes.c_double * MAX_TC),
+        ("tidal_phase", ctypes.c_double * MAX_TC),
+        ("tidal_omega", ctypes.c_double * MAX_TC),
+        ("inflow_velocity", ctypes.c_double),
+        ("discharge_flux", ctypes.c_double),
+        ("clamped_elevation", ctypes.c_double),
+    ]
+
+
+class RkgError(RuntimeError):
+    def __init__(self, fn: str, code: int, detail: str = ""):
+        self.code = code
+        msg = f"{fn} returned {code} ({ERRORS.get(code, 'error')})"
+        if detail:
+            msg += f": {detail}"
+        super().__init__(msg)
+
+
+_lib = None
+
+# Every symbol declared in include/rakali_amd.h, with (restype, argtypes).
+_P = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "rkg_config_default": (ctypes.c_int, [ctypes.POINTER(Config)]),
+    "rkg_create": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.POINTER(_P)]),
+    "rkg_destroy": (None, [ctypes.POINTER(_P)]),
+    "rkg_upload": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int]),
+    "rkg_download": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int]),
+    "rkg_upload_device": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int]),
+    "rkg_download_device": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int]),
+    "rkg_field_device_ptr": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P),
+                                            ctypes.POINTER(ctypes.c_longlong)]),
+    "rkg_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_advance": (ctypes.c_int, [_P, ctypes.c_double, ctypes.POINTER(ctypes.c_longlong)]),
+    "rkg_get_clock": (ctypes.c_int, [_P, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "rkg_set_clock": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_longlong]),
+    "rkg_total_mass": (ctypes.c_int, [_P, _dp]),
+    "rkg_sync": (ctypes.c_int, [_P]),
+    "rkg_set_tile_rows": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_get_tile_rows": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "rkg_last_error": (ctypes.c_char_p, []),
+    "rkg_math_eval": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int]),
+}
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m rakali_amd.build` "
+            "(hipcc, gfx950). There is no CPU fallback.")
+    L = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(fn: str, code: int):
+    if code != RKG_OK:
+        detail = load().rkg_last_error()
+        raise RkgError(fn, code, detail.decode() if detail else "")

@@ -1,0 +1,178 @@
+// crmath.hpp -- x**(1.0/3.0), sin and cos evaluated identically on the
+// host and on gfx950, rounded to nearest.
+//
+// The reference evaluates h**(1.0/3.0) in Manning friction
+// (rki_kernel_fused.F90:268), sin/cos for the tidal surface
+// (rki_boundary.F90:96-100) and cos/sin for Coriolis
+// (rki_kernel_fused.F90:212-213) with the platform libm, whose last-ulp
+// rounding is platform specific (ROCm's ocml pow and glibc's pow disagree
+// for a large fraction of inputs; glibc itself misrounds pow(x, 1/3) for
+// ~7e-4 of inputs).  These routines use only IEEE +, -, *, /, fma and
+// exact bit manipulation, in double-double where it matters, so they give
+// the same bits on every IEEE machine and the correctly rounded result
+// except within ~1e-26 relative of a rounding midpoint.
+//
+// pow13: cube root seeded by exponent splitting and a quadratic fit, four
+// Newton steps, one double-double residual correction, then the factor
+// exp(-(2^-54/3) ln x) because the double 1.0/3.0 is 1/3 - 2^-54/3.
+// sin/cos: Cody-Waite reduction by pi/2 held in three doubles, Taylor
+// series to x^25 / x^26 in double-double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rk {
+namespace crm {
+
+struct dd {
+    double hi, lo;
+};
+
+__host__ __device__ __forceinline__ double bits2d(uint64_t u) {
+    union { uint64_t u; double d; } v;
+    v.u = u;
+    return v.d;
+}
+__host__ __device__ __forceinline__ uint64_t d2bits(double d) {
+    union { uint64_t u; double d; } v;
+    v.d = d;
+    return v.u;
+}
+
+__host__ __device__ __forceinline__ dd two_sum(double a, double b) {
+    const double s = a + b;
+    const double bb = s - a;
+    return dd{s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ __forceinline__ dd fast_two_sum(double a, double b) {
+    const double s = a + b;
+    return dd{s, b - (s - a)};
+}
+__host__ __device__ __forceinline__ dd add(dd x, dd y) {
+    const dd s = two_sum(x.hi, y.hi);
+    const double e = s.lo + (x.lo + y.lo);
+    return fast_two_sum(s.hi, e);
+}
+__host__ __device__ __forceinline__ dd mul(dd x, dd y) {
+    const double p = x.hi * y.hi;
+    double e = fma(x.hi, y.hi, -p);
+    e = e + (x.hi * y.lo + x.lo * y.hi);
+    return fast_two_sum(p, e);
+}
+
+// ln x to ~1e-11 relative for normal x > 0 (atanh series).
+__host__ __device__ __forceinline__ double log_approx(double x) {
+    const uint64_t u = d2bits(x);
+    int e = (int)((u >> 52) & 0x7ff) - 1023;
+    double m = bits2d((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double poly = 1.0 / 11.0;
+    poly = 1.0 / 9.0 + s2 * poly;
+    poly = 1.0 / 7.0 + s2 * poly;
+    poly = 1.0 / 5.0 + s2 * poly;
+    poly = 1.0 / 3.0 + s2 * poly;
+    poly = 1.0 + s2 * poly;
+    return (double)e * 0.6931471805599453 + 2.0 * s * poly;
+}
+
+// x**(1.0/3.0) for normal x > 0.
+__host__ __device__ __forceinline__ double pow13(double x) {
+    const uint64_t u = d2bits(x);
+    const int e = (int)((u >> 52) & 0x7ff) - 1023;
+    const double m = bits2d((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    const int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);  // floor(e / 3)
+    const int r = e - 3 * q;
+    const double y = m * (double)(1 << r);  // [1, 8)
+    double c = 0.62490765 + m * (0.43556508 + m * (-0.05930307));
+    c = c * (r == 0 ? 1.0 : (r == 1 ? 1.2599210498948732 : 1.5874010519681994));
+    for (int it = 0; it < 4; ++it) c = (2.0 * c + y / (c * c)) / 3.0;
+    const double r0 = c * bits2d((uint64_t)(q + 1023) << 52);
+    const double p = r0 * r0;
+    const double pl = fma(r0, r0, -p);
+    const double t3 = p * r0;
+    const double t3l = fma(p, r0, -t3);
+    const double res = (x - t3) - (t3l + pl * r0);
+    const double corr = res / (3.0 * p);
+    const double t = -(log_approx(x) * (1.0 / 3.0)) * 5.551115123125783e-17;  // 2^-54
+    return r0 + (corr + r0 * t);
+}
+
+// 1/n! as double-double, n = 0..26.
+__host__ __device__ __forceinline__ dd inv_fact(int n) {
+    switch (n) {
+    case 0: return dd{1.0, 0.0};
+    case 1: return dd{1.0, 0.0};
+    case 2: return dd{0.5, 0.0};
+    case 3: return dd{0x1.5555555555555p-3, 0x1.5555555555555p-57};
+    case 4: return dd{0x1.5555555555555p-5, 0x1.5555555555555p-59};
+    case 5: return dd{0x1.1111111111111p-7, 0x1.1111111111111p-63};
+    case 6: return dd{0x1.6c16c16c16c17p-10, -0x1.f49f49f49f49fp-65};
+    case 7: return dd{0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-73};
+    case 8: return dd{0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-76};
+    case 9: return dd{0x1.71de3a556c734p-19, -0x1.c154f8ddc6c00p-73};
+    case 10: return dd{0x1.27e4fb7789f5cp-22, 0x1.cbbc05b4fa99ap-76};
+    case 11: return dd{0x1.ae64567f544e4p-26, -0x1.c062e06d1f209p-80};
+    case 12: return dd{0x1.1eed8eff8d898p-29, -0x1.2aec959e14c06p-83};
+    case 13: return dd{0x1.6124613a86d09p-33, 0x1.f28e0cc748ebep-87};
+    case 14: return dd{0x1.93974a8c07c9dp-37, 0x1.05d6f8a2efd1fp-92};
+    case 15: return dd{0x1.ae7f3e733b81fp-41, 0x1.1d8656b0ee8cbp-97};
+    case 16: return dd{0x1.ae7f3e733b81fp-45, 0x1.1d8656b0ee8cbp-101};
+    case 17: return dd{0x1.952c77030ad4ap-49, 0x1.ac981465ddc6cp-103};
+    case 18: return dd{0x1.6827863b97d97p-53, 0x1.eec01221a8b0bp-107};
+    case 19: return dd{0x1.2f49b46814157p-57, 0x1.2650f61dbdcb4p-112};
+    case 20: return dd{0x1.e542ba4020225p-62, 0x1.ea72b4afe3c2fp-120};
+    case 21: return dd{0x1.71b8ef6dcf572p-66, -0x1.d043ae40c4647p-120};
+    case 22: return dd{0x1.0ce396db7f853p-70, -0x1.aebcdbd20331cp-124};
+    case 23: return dd{0x1.761b41316381ap-75, -0x1.3423c7d91404fp-130};
+    case 24: return dd{0x1.f2cf01972f578p-80, -0x1.9ada5fcc1ab14p-135};
+    case 25: return dd{0x1.3f3ccdd165fa9p-84, -0x1.58ddadf344487p-139};
+    default: return dd{0x1.88e85fc6a4e5ap-89, -0x1.71c37ebd16540p-143};
+    }
+}
+
+// x - k*pi/2 in double-double, |k| < 2^30.
+__host__ __device__ __forceinline__ dd reduce(double x, double k) {
+    const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c07p-54, P3 = -0x1.f1976b7ed8fbcp-110;
+    const double p = k * P1;
+    const double pl = fma(k, P1, -p);
+    dd r = two_sum(x, -p);
+    r = fast_two_sum(r.hi, r.lo - pl);
+    const double q = k * P2;
+    const double ql = fma(k, P2, -q);
+    r = add(r, dd{-q, -ql});
+    return add(r, dd{-(k * P3), 0.0});
+}
+
+// Taylor series of sin (odd = 1) or cos (odd = 0); term n has sign
+// (-1)^((n - odd) / 2).
+__host__ __device__ __forceinline__ dd series(dd r, int odd) {
+    const dd r2 = mul(r, r);
+    int n = odd ? 25 : 26;
+    dd poly = inv_fact(n);
+    if (((n - odd) / 2) % 2 == 1) { poly.hi = -poly.hi; poly.lo = -poly.lo; }
+    for (n = n - 2; n >= 0; n -= 2) {
+        dd c = inv_fact(n);
+        if (((n - odd) / 2) % 2 == 1) { c.hi = -c.hi; c.lo = -c.lo; }
+        poly = add(c, mul(r2, poly));
+    }
+    return odd ? mul(r, poly) : poly;
+}
+
+__host__ __device__ __forceinline__ double sincos_one(double x, int want_cos) {
+    const double k = rint(x * 0x1.45f306dc9c883p-1);  // x * 2/pi
+    const dd r = reduce(x, k);
+    const long long ki = (long long)k;
+    int quad = (int)(((ki % 4) + 4) % 4);
+    if (want_cos) quad = (quad + 1) % 4;
+    const dd v = series(r, (quad % 2 == 0) ? 1 : 0);
+    return (quad >= 2) ? -v.hi : v.hi;
+}
+
+__host__ __device__ __forceinline__ double sin(double x) { return sincos_one(x, 0); }
+__host__ __device__ __forceinline__ double cos(double x) { return sincos_one(x, 1); }
+
+}  // namespace crm
+}  // namespace rk

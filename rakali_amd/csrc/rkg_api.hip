@@ -1,0 +1,567 @@
+// rkg_api.hip -- stage kernels, CFL/mass reductions and the rkg_* C ABI.
+//
+// One SSP-RK2 step is two launches:
+//   stage 1: dt from the CFL speed of U^n (carried in the control slot),
+//            U* = clampfriction(U^n + dt L(U^n))                -> B
+//   stage 2: U^{n+1} = clamp(0.5 (U^n + clampfriction(U* + dt L(U*)))) -> A,
+//            plus the max CFL speed of U^{n+1} for the next step's dt
+//            (wave max + one atomic per workgroup).
+// This is the data flow of rki_solver.F90:718-815 with the boundary fill
+// folded into the tile loader and compute_timestep folded into the
+// stage-2 epilogue.  dt and t never leave the device between steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rakali_amd.h"
+#include "swe_kernels.hpp"
+
+using namespace rk;
+
+namespace {
+
+constexpr unsigned GRID_MAGIC = 0x72616B47u;  // 'rakG'
+constexpr unsigned GRID_DEAD = 0xDEADBEEFu;
+constexpr int MODE_STEP = 0;
+constexpr int MODE_ADVANCE = 1;
+
+thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+#define HIP_TRY(expr)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            set_error("%s failed: %s", #expr, hipGetErrorString(e_));          \
+            return RKG_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+// dt of a step from the control slot of the state it starts from:
+// rki_solver.F90:728-745 with rki_api.F90:377 (step) / :402 (advance).
+__device__ __forceinline__ double step_dt(const GridDesc& d, const CtrlSlot& cp, double dt_max) {
+    if (d.dt_fixed > 0.0) return dmin(d.dt_fixed, dt_max);
+    if (cp.step_count % d.cfl_interval == 0 || cp.dt <= 0.0) {
+        const double max_speed = __longlong_as_double((long long)cp.speed_bits);
+        double dt;
+        if (max_speed > 0.0)
+            dt = d.cfl / max_speed;
+        else
+            dt = d.cfl * dmin(d.dx, d.dy) / sqrt(d.g * DRY_TOLERANCE);
+        dt = dt * 1.0;  // ssp_coeff of RK2
+        return dmin(dt, dt_max);
+    }
+    return cp.dt;
+}
+
+__device__ __forceinline__ SourceTerms make_src(const GridDesc& d, double dt) {
+    SourceTerms s;
+    s.g = d.g;
+    s.n_sq = d.manning_n * d.manning_n;
+    s.has_friction = d.manning_n > 0.0;
+    s.has_coriolis = fabs(d.coriolis_f) > 1.0e-20;
+    s.cosf = 0.0;
+    s.sinf = 0.0;
+    if (s.has_coriolis) {
+        s.cosf = crm::cos(d.coriolis_f * dt);
+        s.sinf = crm::sin(d.coriolis_f * dt);
+    }
+    return s;
+}
+
+__device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, int seg) {
+    const int c0 = strip * WAVE;
+    const int j0 = seg * d.ty;
+    const int j1 = min(j0 + d.ty, d.ny);
+    return c0 >= 2 && c0 + WAVE + 1 < d.nx && j0 >= 2 && j1 + 1 < d.ny;
+}
+
+template <int STAGE, bool HLLC>
+__global__ __launch_bounds__(WAVE) void k_stage(GridDesc d, int p, int mode, double tlim) {
+    __shared__ TileLds L;
+    StageCtx s;
+    if (STAGE == 1) {
+        const CtrlSlot cp = d.ctrl[p];
+        const bool active = (mode == MODE_STEP) || (cp.t < tlim - 1.0e-10);
+        const double dt_max = tlim - cp.t;
+        const double dt = active ? step_dt(d, cp, dt_max) : cp.dt;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            CtrlSlot* q = &d.ctrl[1 - p];
+            q->t = active ? cp.t + dt : cp.t;
+            q->dt = dt;
+            q->step_count = active ? cp.step_count + 1 : cp.step_count;
+            q->speed_bits = active ? 0ull : cp.speed_bits;
+            q->active = active ? 1 : 0;
+        }
+        if (!active) return;
+        s.ih = d.ah; s.ihu = d.ahu; s.ihv = d.ahv;
+        s.oh = d.bh; s.ohu = d.bhu; s.ohv = d.bhv;
+        s.dt = dt;
+        s.eta_tidal = tidal_eta(d, cp.t);
+    } else {
+        const CtrlSlot cq = d.ctrl[1 - p];
+        if (!cq.active) return;
+        s.ih = d.bh; s.ihu = d.bhu; s.ihv = d.bhv;
+        s.oh = d.ah; s.ohu = d.ahu; s.ohv = d.ahv;
+        s.dt = cq.dt;
+        s.eta_tidal = tidal_eta(d, cq.t);  // bc%time = t + dt
+    }
+    s.src = make_src(d, s.dt);
+
+    const int ntiles = d.nstrips * d.nsegs;
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int strip = tile % d.nstrips;
+    const int seg = tile / d.nstrips;
+    double smax;
+    if (tile_is_interior(d, strip, seg))
+        smax = process_tile<STAGE, HLLC, false>(d, s, L, strip, seg);
+    else
+        smax = process_tile<STAGE, HLLC, true>(d, s, L, strip, seg);
+    if (STAGE == 2) {
+        smax = wave_max(smax);
+        if (threadIdx.x == 0 && smax > 0.0)
+            atomicMax(&d.ctrl[1 - p].speed_bits, (unsigned long long)__double_as_longlong(smax));
+    }
+}
+
+// Resets control slot p to time t / step count n with an empty speed max.
+__global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long long n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctrl[p].t = t;
+        ctrl[p].dt = dt;
+        ctrl[p].step_count = n;
+        ctrl[p].speed_bits = 0ull;
+        ctrl[p].active = 1;
+    }
+}
+
+// Max CFL speed of the state A into slot p (compute_timestep's reduction,
+// rki_kernel_timestep.F90:53-70).
+__global__ __launch_bounds__(256) void k_init_speed(GridDesc d, int p) {
+    double m = 0.0;
+    const long long n = (long long)d.nx * d.ny;
+    for (long long k = blockIdx.x * 256ll + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
+        const long long o = (long long)j * d.pitch + i;
+        m = dmax(m, cfl_speed(d.g, d.dx, d.dy, d.ah[o], d.ahu[o], d.ahv[o]));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && m > 0.0)
+        atomicMax(&d.ctrl[p].speed_bits, (unsigned long long)__double_as_longlong(m));
+}
+
+// Mass partials: block b sums rows b, b + gridDim.x, ... in a fixed order
+// (compute_total_mass, rki_init.F90:287-317).
+__global__ __launch_bounds__(256) void k_mass_partial(GridDesc d, double* partial) {
+    __shared__ double red[256];
+    const double area = d.dx * d.dy;
+    double acc = 0.0;
+    for (int j = blockIdx.x; j < d.ny; j += gridDim.x) {
+        for (int i = threadIdx.x; i < d.nx; i += 256) {
+            const double h = d.ah[(long long)j * d.pitch + i];
+            if (h > 0.0 && h < 1.0e6) acc = acc + h * area;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_mass_final(const double* partial, int n, double* out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc = acc + partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+constexpr int MASS_BLOCKS = 512;
+
+__global__ void k_math_eval(int fn, const double* x, double* y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = fn == 0 ? crm::pow13(x[i]) : (fn == 1 ? crm::sin(x[i]) : crm::cos(x[i]));
+}
+
+}  // namespace
+
+struct rkg_grid {
+    unsigned magic;
+    rkg_config cfg;
+    int device;
+    hipStream_t stream;
+    bool own_stream;
+    long long pitch;
+    size_t plane;        // elements per field plane
+    double* buf;         // 7 planes: A(h,hu,hv) B(h,hu,hv) bed
+    CtrlSlot* ctrl;      // device [2]
+    CtrlSlot* ctrl_host; // pinned [2]
+    double* mass_buf;    // device [MASS_BLOCKS + 1]
+    double* mass_host;   // pinned [1]
+    int parity;
+    int ty;
+    bool speed_stale;
+};
+
+namespace {
+
+bool valid(const rkg_grid* g) { return g != nullptr && g->magic == GRID_MAGIC; }
+
+double* plane_ptr(rkg_grid* g, int k) { return g->buf + (size_t)k * g->plane; }
+
+double* field_ptr(rkg_grid* g, int field) {
+    switch (field) {
+    case RKG_FIELD_H: return plane_ptr(g, 0);
+    case RKG_FIELD_HU: return plane_ptr(g, 1);
+    case RKG_FIELD_HV: return plane_ptr(g, 2);
+    case RKG_FIELD_B: return plane_ptr(g, 6);
+    default: return nullptr;
+    }
+}
+
+GridDesc make_desc(rkg_grid* g) {
+    GridDesc d;
+    memset(&d, 0, sizeof d);
+    const rkg_config& c = g->cfg;
+    d.nx = c.nx; d.ny = c.ny; d.pitch = g->pitch;
+    d.dx = c.dx; d.dy = c.dy;
+    d.ah = plane_ptr(g, 0); d.ahu = plane_ptr(g, 1); d.ahv = plane_ptr(g, 2);
+    d.bh = plane_ptr(g, 3); d.bhu = plane_ptr(g, 4); d.bhv = plane_ptr(g, 5);
+    d.bed = plane_ptr(g, 6);
+    d.g = c.g; d.cfl = c.cfl; d.manning_n = c.manning_n; d.coriolis_f = c.coriolis_f;
+    d.use_hllc = c.use_hllc;
+    for (int s = 0; s < 4; ++s) d.bc[s] = c.bc[s];
+    d.tidal_amplitude = c.tidal_amplitude; d.tidal_period = c.tidal_period;
+    d.n_tc = c.n_tidal_constituents;
+    for (int n = 0; n < RKG_MAX_TIDAL_CONSTITUENTS; ++n) {
+        d.tc_amp[n] = c.tidal_amp[n]; d.tc_phase[n] = c.tidal_phase[n]; d.tc_omega[n] = c.tidal_omega[n];
+    }
+    d.inflow_velocity = c.inflow_velocity; d.discharge_flux = c.discharge_flux;
+    d.clamped_elevation = c.clamped_elevation;
+    d.dt_fixed = c.dt_fixed; d.t_end = c.t_end;
+    d.cfl_interval = c.cfl_interval;
+    d.ty = g->ty;
+    d.nstrips = (c.nx + WAVE - 1) / WAVE;
+    d.nsegs = (c.ny + g->ty - 1) / g->ty;
+    d.ctrl = g->ctrl;
+    return d;
+}
+
+int refresh_speed(rkg_grid* g) {
+    GridDesc d = make_desc(g);
+    const CtrlSlot& cur = g->ctrl_host[g->parity];
+    (void)cur;
+    // keep t / dt / step_count of the current slot, clear its speed max
+    HIP_TRY(hipMemcpyAsync(g->ctrl_host, g->ctrl, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    const CtrlSlot c = g->ctrl_host[g->parity];
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, c.t, c.dt,
+                       c.step_count);
+    const long long n = (long long)g->cfg.nx * g->cfg.ny;
+    int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_init_speed, dim3(blocks), dim3(256), 0, g->stream, d, g->parity);
+    HIP_TRY(hipGetLastError());
+    g->speed_stale = false;
+    return RKG_OK;
+}
+
+int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
+    if (g->speed_stale) {
+        int rc = refresh_speed(g);
+        if (rc) return rc;
+    }
+    GridDesc d = make_desc(g);
+    const int ntiles = d.nstrips * d.nsegs;
+    for (int k = 0; k < n; ++k) {
+        if (g->cfg.use_hllc) {
+            hipLaunchKernelGGL((k_stage<1, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+            hipLaunchKernelGGL((k_stage<2, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        } else {
+            hipLaunchKernelGGL((k_stage<1, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+            hipLaunchKernelGGL((k_stage<2, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        }
+        g->parity ^= 1;
+    }
+    HIP_TRY(hipGetLastError());
+    return RKG_OK;
+}
+
+int read_clock(rkg_grid* g, CtrlSlot* out) {
+    HIP_TRY(hipMemcpyAsync(g->ctrl_host, g->ctrl, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    *out = g->ctrl_host[g->parity];
+    return RKG_OK;
+}
+
+int check_config(const rkg_config* c) {
+    if (c->nx < 2 || c->ny < 2) { set_error("nx and ny must be >= 2 (got %d, %d)", c->nx, c->ny); return RKG_ERR_RANGE; }
+    if (!(c->dx > 0.0) || !(c->dy > 0.0)) { set_error("dx and dy must be positive"); return RKG_ERR_RANGE; }
+    if (c->cfl_interval < 1) { set_error("cfl_interval must be >= 1"); return RKG_ERR_RANGE; }
+    if (c->n_tidal_constituents < 0 || c->n_tidal_constituents > RKG_MAX_TIDAL_CONSTITUENTS) {
+        set_error("n_tidal_constituents out of range"); return RKG_ERR_RANGE;
+    }
+    for (int s = 0; s < 4; ++s) {
+        switch (c->bc[s]) {
+        case RKG_BC_WALL: case RKG_BC_OPEN: case RKG_BC_TIDAL: case RKG_BC_INFLOW:
+        case RKG_BC_DISCHARGE: case RKG_BC_CLAMPED: case RKG_BC_SPONGE: break;
+        default:
+            set_error("boundary tag %d on edge %d is not implemented (nested/chapman)", c->bc[s], s);
+            return RKG_ERR_UNSUPPORTED;
+        }
+    }
+    return RKG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rkg_config_default(rkg_config* c) {
+    if (!c) return RKG_ERR_RANGE;
+    memset(c, 0, sizeof *c);
+    c->nx = 200; c->ny = 1;
+    c->dx = 1.0; c->dy = 1.0;
+    c->g = 9.80665;
+    c->cfl = 0.45;
+    c->t_end = 1.0;
+    c->cfl_interval = 1;
+    c->use_hllc = 0;
+    for (int s = 0; s < 4; ++s) c->bc[s] = RKG_BC_WALL;
+    c->tidal_period = 44712.0;
+    return RKG_OK;
+}
+
+int rkg_create(const rkg_config* cfg, void* stream, rkg_grid** out) {
+    if (!cfg || !out) { set_error("null argument"); return RKG_ERR_RANGE; }
+    *out = nullptr;
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    rkg_grid* g = new rkg_grid();
+    memset(g, 0, sizeof *g);
+    g->cfg = *cfg;
+    g->ty = 32;
+    HIP_TRY(hipGetDevice(&g->device));
+    if (stream) {
+        g->stream = (hipStream_t)stream;
+        g->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { set_error("hipStreamCreate: %s", hipGetErrorString(e)); delete g; return RKG_ERR_HIP; }
+        g->own_stream = true;
+    }
+    long long pitch = ((long long)cfg->nx + 15) / 16 * 16;
+    if (pitch % 1024 == 0) pitch += 32;  // break power-of-two row strides
+    g->pitch = pitch;
+    g->plane = (size_t)pitch * (size_t)cfg->ny;
+    hipError_t e = hipMalloc((void**)&g->buf, 7 * g->plane * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&g->ctrl, 2 * sizeof(CtrlSlot));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&g->ctrl_host, 2 * sizeof(CtrlSlot), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&g->mass_buf, (MASS_BLOCKS + 1) * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&g->mass_host, sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemsetAsync(g->buf, 0, 7 * g->plane * sizeof(double), g->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g->ctrl, 0, 2 * sizeof(CtrlSlot), g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) {
+        set_error("device allocation failed: %s", hipGetErrorString(e));
+        g->magic = GRID_MAGIC;
+        rkg_destroy(&g);
+        return RKG_ERR_HIP;
+    }
+    memset(g->ctrl_host, 0, 2 * sizeof(CtrlSlot));
+    g->parity = 0;
+    g->speed_stale = true;
+    g->magic = GRID_MAGIC;
+    *out = g;
+    return RKG_OK;
+}
+
+void rkg_destroy(rkg_grid** pg) {
+    if (!pg || !*pg) return;
+    rkg_grid* g = *pg;
+    if (g->magic != GRID_MAGIC) { *pg = nullptr; return; }
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->buf) (void)hipFree(g->buf);
+    if (g->ctrl) (void)hipFree(g->ctrl);
+    if (g->mass_buf) (void)hipFree(g->mass_buf);
+    if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
+    if (g->mass_host) (void)hipHostFree(g->mass_host);
+    if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
+    g->magic = GRID_DEAD;
+    delete g;
+    *pg = nullptr;
+}
+
+static int copy_field(rkg_grid* g, int field, double* host_or_dev, int ld, bool upload, hipMemcpyKind kind) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    double* f = field_ptr(g, field);
+    if (!f) { set_error("unknown field %d", field); return RKG_ERR_RANGE; }
+    if (!host_or_dev || ld < g->cfg.nx) { set_error("bad buffer or leading dimension"); return RKG_ERR_RANGE; }
+    const size_t w = (size_t)g->cfg.nx * sizeof(double);
+    if (upload) {
+        HIP_TRY(hipMemcpy2DAsync(f, (size_t)g->pitch * sizeof(double), host_or_dev, (size_t)ld * sizeof(double), w,
+                                 (size_t)g->cfg.ny, kind, g->stream));
+        g->speed_stale = true;
+    } else {
+        HIP_TRY(hipMemcpy2DAsync(host_or_dev, (size_t)ld * sizeof(double), f, (size_t)g->pitch * sizeof(double), w,
+                                 (size_t)g->cfg.ny, kind, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return RKG_OK;
+}
+
+int rkg_upload(rkg_grid* g, int field, const double* src, int ld) {
+    return copy_field(g, field, const_cast<double*>(src), ld, true, hipMemcpyHostToDevice);
+}
+int rkg_download(rkg_grid* g, int field, double* dst, int ld) {
+    return copy_field(g, field, dst, ld, false, hipMemcpyDeviceToHost);
+}
+int rkg_upload_device(rkg_grid* g, int field, const double* src, int ld) {
+    return copy_field(g, field, const_cast<double*>(src), ld, true, hipMemcpyDeviceToDevice);
+}
+int rkg_download_device(rkg_grid* g, int field, double* dst, int ld) {
+    return copy_field(g, field, dst, ld, false, hipMemcpyDeviceToDevice);
+}
+
+int rkg_field_device_ptr(rkg_grid* g, int field, double** ptr, long long* pitch) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    double* f = field_ptr(g, field);
+    if (!f || !ptr || !pitch) return RKG_ERR_RANGE;
+    *ptr = f;
+    *pitch = g->pitch;
+    return RKG_OK;
+}
+
+int rkg_step(rkg_grid* g, int n) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (n < 0) return RKG_ERR_RANGE;
+    return launch_steps(g, n, MODE_STEP, g->cfg.t_end);
+}
+
+int rkg_advance(rkg_grid* g, double target, long long* steps_taken) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    CtrlSlot c;
+    int rc = read_clock(g, &c);
+    if (rc) return rc;
+    const long long start = c.step_count;
+    // Launch in chunks; steps past the target are no-ops on the device.
+    int chunk = 8;
+    while (c.t < target - 1.0e-10) {
+        rc = launch_steps(g, chunk, MODE_ADVANCE, target);
+        if (rc) return rc;
+        rc = read_clock(g, &c);
+        if (rc) return rc;
+        if (c.t < target - 1.0e-10 && c.dt > 0.0) {
+            const double need = (target - c.t) / c.dt + 2.0;
+            chunk = (int)std::min(4096.0, std::max(8.0, need));
+        } else {
+            chunk = std::min(chunk * 2, 4096);
+        }
+    }
+    if (steps_taken) *steps_taken = c.step_count - start;
+    return RKG_OK;
+}
+
+int rkg_get_clock(rkg_grid* g, double* t, double* dt, long long* step_count) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    CtrlSlot c;
+    int rc = read_clock(g, &c);
+    if (rc) return rc;
+    if (t) *t = c.t;
+    if (dt) *dt = c.dt;
+    if (step_count) *step_count = c.step_count;
+    return RKG_OK;
+}
+
+int rkg_set_clock(rkg_grid* g, double t, long long step_count) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    CtrlSlot c;
+    int rc = read_clock(g, &c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, t, c.dt, step_count);
+    HIP_TRY(hipGetLastError());
+    g->speed_stale = true;
+    return RKG_OK;
+}
+
+int rkg_total_mass(rkg_grid* g, double* mass) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!mass) return RKG_ERR_RANGE;
+    GridDesc d = make_desc(g);
+    const int blocks = std::min(MASS_BLOCKS, g->cfg.ny);
+    hipLaunchKernelGGL(k_mass_partial, dim3(blocks), dim3(256), 0, g->stream, d, g->mass_buf);
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, g->stream, g->mass_buf, blocks,
+                       g->mass_buf + MASS_BLOCKS);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(g->mass_host, g->mass_buf + MASS_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    *mass = *g->mass_host;
+    return RKG_OK;
+}
+
+int rkg_sync(rkg_grid* g) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return RKG_OK;
+}
+
+int rkg_set_tile_rows(rkg_grid* g, int rows) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (rows < 1 || rows > WAVE) { set_error("tile rows must be in 1..64"); return RKG_ERR_RANGE; }
+    g->ty = rows;
+    return RKG_OK;
+}
+
+int rkg_get_tile_rows(rkg_grid* g, int* rows) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!rows) return RKG_ERR_RANGE;
+    *rows = g->ty;
+    return RKG_OK;
+}
+
+const char* rkg_last_error(void) { return g_last_error.c_str(); }
+
+int rkg_math_eval(int fn, const double* x, double* y, int n, int on_device) {
+    if (fn < 0 || fn > 2 || n < 0 || (n > 0 && (!x || !y))) { set_error("bad arguments"); return RKG_ERR_RANGE; }
+    if (!on_device) {
+        for (int i = 0; i < n; ++i)
+            y[i] = fn == 0 ? crm::pow13(x[i]) : (fn == 1 ? crm::sin(x[i]) : crm::cos(x[i]));
+        return RKG_OK;
+    }
+    if (n == 0) return RKG_OK;
+    double* d = nullptr;
+    HIP_TRY(hipMalloc((void**)&d, 2 * (size_t)n * sizeof(double)));
+    hipError_t e = hipMemcpy(d, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_math_eval, dim3((n + 255) / 256), dim3(256), 0, 0, fn, d, d + n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(y, d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) { set_error("rkg_math_eval: %s", hipGetErrorString(e)); return RKG_ERR_HIP; }
+    return RKG_OK;
+}
+
+}  // extern "C"

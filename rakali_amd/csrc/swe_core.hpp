@@ -1,0 +1,303 @@
+// swe_core.hpp -- per-face and per-cell arithmetic of the structured 2-D
+// shallow-water step, as gfx950 device functions.
+//
+// The step is Rakali's SSP-RK2 barotropic step (rki_solver.F90:761-815):
+// Audusse hydrostatic reconstruction with minmod MUSCL slopes, HLL/HLLC
+// Riemann fluxes with Ritter dry-front speeds, the well-balanced pressure
+// correction, and the fused Euler update with clamp, KP07
+// desingularization, Froude backstop, semi-implicit Manning and Coriolis.
+//
+// Arithmetic is written expression-for-expression in the reference's
+// evaluation order and built with -ffp-contract=off, so each face and cell
+// value is the IEEE result of the same operation sequence.  The reference
+// evaluates every interior face twice (once from each cell,
+// rki_kernel_flux.F90:371-372) and both evaluations are bit-identical,
+// because a face depends only on the four cells across it; `face()` below
+// evaluates each face once and hands it to both cells.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "crmath.hpp"
+
+namespace rk {
+
+// rki_constants.F90:53,59,68-73,120
+constexpr double DRY_TOLERANCE = 1.0e-6;
+constexpr double THIN_LAYER_THRESHOLD = 1.0e-3;
+constexpr double DESING_EPS = 1.5e-4;
+constexpr double FROUDE_BACKSTOP = 50.0;
+constexpr double TWO_PI = 6.283185307179586;  // 8*atan(1), rki_constants.F90:114
+constexpr double SQRT2 = 1.4142135623730951;  // sqrt(2.0_wp) folded
+
+// Boundary tags (rki_boundary_types.F90:18-32).
+enum : int { BC_WALL = 1, BC_OPEN = 2, BC_TIDAL = 3, BC_NESTED = 4, BC_INFLOW = 5,
+             BC_DISCHARGE = 6, BC_CLAMPED = 7, BC_SPONGE = 8, BC_CHAPMAN = 9 };
+
+struct Cell {
+    double h, hu, hv, b;
+};
+
+// One face's contribution to its two cells.  a = h_hyd_r^2 - h_R^2 is the
+// right cell's "left" pressure term; c = h_hyd_l^2 and d = h_L^2 are the
+// left cell's "right" terms (rki_kernel_flux.F90:361-369 evaluates
+// ((hyd_left^2 - pre_left^2) - hyd_right^2) + pre_right^2).
+struct Face {
+    double fh, fhu, fhv, a, c, d;
+};
+
+__device__ __forceinline__ double fmax0(double x) { return x > 0.0 ? x : 0.0; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+// x**4 as LLVM flang lowers it for a run-time base: ((x*x)*x)*x.
+__device__ __forceinline__ double pow4(double x) { return ((x * x) * x) * x; }
+
+// minmod, rki_kernel_flux.F90:380-393
+__device__ __forceinline__ double minmod(double a, double b) {
+    if (a * b <= 0.0) return 0.0;
+    return (fabs(a) < fabs(b)) ? a : b;
+}
+
+// HLL / HLLC normal flux in x: rki_kernel_flux.F90:410-495 and 584-684.
+template <bool HLLC>
+__device__ __forceinline__ void riemann_x(double g, double h_l, double hu_l, double hv_l,
+                                          double h_r, double hu_r, double hv_r,
+                                          double& f_h, double& f_hu, double& f_hv) {
+    double u_l, v_l, u_r, v_r;
+    if (h_l > DRY_TOLERANCE) { u_l = hu_l / h_l; v_l = hv_l / h_l; } else { u_l = 0.0; v_l = 0.0; }
+    if (h_r > DRY_TOLERANCE) { u_r = hu_r / h_r; v_r = hv_r / h_r; } else { u_r = 0.0; v_r = 0.0; }
+    const double a_l = sqrt(g * fmax0(h_l));
+    const double a_r = sqrt(g * fmax0(h_r));
+    double s_l, s_r;
+    if (h_r <= DRY_TOLERANCE) {          // dry right: Ritter front speed
+        s_l = u_l - a_l;
+        s_r = u_l + 2.0 * a_l;
+    } else if (h_l <= DRY_TOLERANCE) {   // dry left
+        s_l = u_r - 2.0 * a_r;
+        s_r = u_r + a_r;
+    } else {                             // Davis
+        s_l = dmin(u_l - a_l, u_r - a_r);
+        s_r = dmax(u_l + a_l, u_r + a_r);
+    }
+    const double fl_h = hu_l, fl_hu = hu_l * u_l + 0.5 * g * h_l * h_l, fl_hv = hu_l * v_l;
+    const double fr_h = hu_r, fr_hu = hu_r * u_r + 0.5 * g * h_r * h_r, fr_hv = hu_r * v_r;
+    if (s_l >= 0.0) {
+        f_h = fl_h; f_hu = fl_hu; f_hv = fl_hv;
+    } else if (s_r <= 0.0) {
+        f_h = fr_h; f_hu = fr_hu; f_hv = fr_hv;
+    } else {
+        const double denom = 1.0 / (s_r - s_l);
+        f_h = (s_r * fl_h - s_l * fr_h + s_l * s_r * (h_r - h_l)) * denom;
+        f_hu = (s_r * fl_hu - s_l * fr_hu + s_l * s_r * (hu_r - hu_l)) * denom;
+        if (!HLLC) {
+            f_hv = (s_r * fl_hv - s_l * fr_hv + s_l * s_r * (hv_r - hv_l)) * denom;
+        } else {
+            // contact speed, rki_kernel_flux.F90:670-681
+            const double denom_star = h_r * (u_r - s_r) - h_l * (u_l - s_l);
+            double s_star;
+            if (fabs(denom_star) > DRY_TOLERANCE)
+                s_star = (s_l * h_r * (u_r - s_r) - s_r * h_l * (u_l - s_l)) / denom_star;
+            else
+                s_star = 0.5 * (u_l + u_r);
+            f_hv = (s_star >= 0.0) ? f_h * v_l : f_h * v_r;
+        }
+    }
+}
+
+// HLL / HLLC normal flux in y: rki_kernel_flux.F90:497-582 and 686-783.
+// Arguments are (h, hu, hv) of the bottom then the top state, as there.
+template <bool HLLC>
+__device__ __forceinline__ void riemann_y(double g, double h_b, double hu_b, double hv_b,
+                                          double h_t, double hu_t, double hv_t,
+                                          double& g_h, double& g_hu, double& g_hv) {
+    double u_b, v_b, u_t, v_t;
+    if (h_b > DRY_TOLERANCE) { u_b = hu_b / h_b; v_b = hv_b / h_b; } else { u_b = 0.0; v_b = 0.0; }
+    if (h_t > DRY_TOLERANCE) { u_t = hu_t / h_t; v_t = hv_t / h_t; } else { u_t = 0.0; v_t = 0.0; }
+    const double a_b = sqrt(g * fmax0(h_b));
+    const double a_t = sqrt(g * fmax0(h_t));
+    double s_l, s_r;
+    if (h_t <= DRY_TOLERANCE) {
+        s_l = v_b - a_b;
+        s_r = v_b + 2.0 * a_b;
+    } else if (h_b <= DRY_TOLERANCE) {
+        s_l = v_t - 2.0 * a_t;
+        s_r = v_t + a_t;
+    } else {
+        s_l = dmin(v_b - a_b, v_t - a_t);
+        s_r = dmax(v_b + a_b, v_t + a_t);
+    }
+    const double gl_h = hv_b, gl_hu = hu_b * v_b, gl_hv = hv_b * v_b + 0.5 * g * h_b * h_b;
+    const double gr_h = hv_t, gr_hu = hu_t * v_t, gr_hv = hv_t * v_t + 0.5 * g * h_t * h_t;
+    if (s_l >= 0.0) {
+        g_h = gl_h; g_hu = gl_hu; g_hv = gl_hv;
+    } else if (s_r <= 0.0) {
+        g_h = gr_h; g_hu = gr_hu; g_hv = gr_hv;
+    } else {
+        const double denom = 1.0 / (s_r - s_l);
+        g_h = (s_r * gl_h - s_l * gr_h + s_l * s_r * (h_t - h_b)) * denom;
+        g_hv = (s_r * gl_hv - s_l * gr_hv + s_l * s_r * (hv_t - hv_b)) * denom;
+        if (!HLLC) {
+            g_hu = (s_r * gl_hu - s_l * gr_hu + s_l * s_r * (hu_t - hu_b)) * denom;
+        } else {
+            const double denom_star = h_t * (v_t - s_r) - h_b * (v_b - s_l);
+            double s_star;
+            if (fabs(denom_star) > DRY_TOLERANCE)
+                s_star = (s_l * h_t * (v_t - s_r) - s_r * h_b * (v_b - s_l)) / denom_star;
+            else
+                s_star = 0.5 * (v_b + v_t);
+            g_hu = (s_star >= 0.0) ? g_h * u_b : g_h * u_t;
+        }
+    }
+}
+
+// One face between cell L = m1 and cell R = p0, from the four cells
+// m2, m1, p0, p1 along the face normal (YDIR selects the y-face Riemann
+// solver; the reconstruction is the same in both directions).
+// rki_kernel_flux.F90:158-224 (x left face) restated face-centrically:
+// the left cell's slope uses (m2, m1, p0) and the right cell's (m1, p0, p1).
+template <bool HLLC, bool YDIR>
+__device__ __forceinline__ Face face(double g, const Cell& m2, const Cell& m1, const Cell& p0,
+                                     const Cell& p1) {
+    const double e_m2 = m2.h + m2.b;
+    const double e_m1 = m1.h + m1.b;
+    const double e_p0 = p0.h + p0.b;
+    const double e_p1 = p1.h + p1.b;
+
+    // right cell (p0) slope toward its left face: eta_R = eta_c - 0.5*seta
+    double s_eta, s_hu, s_hv;
+    if (p0.h < THIN_LAYER_THRESHOLD || m1.h < DRY_TOLERANCE || p1.h < DRY_TOLERANCE) {
+        s_eta = 0.0; s_hu = 0.0; s_hv = 0.0;
+    } else {
+        s_eta = minmod(e_p0 - e_m1, e_p1 - e_p0);
+        s_hu = minmod(p0.hu - m1.hu, p1.hu - p0.hu);
+        s_hv = minmod(p0.hv - m1.hv, p1.hv - p0.hv);
+    }
+    const double eta_R = e_p0 - 0.5 * s_eta;
+    const double hu_R = p0.hu - 0.5 * s_hu;
+    const double hv_R = p0.hv - 0.5 * s_hv;
+    const double h_R = fmax0(eta_R - p0.b);
+
+    // left cell (m1) slope toward its right face
+    if (m1.h < THIN_LAYER_THRESHOLD || m2.h < DRY_TOLERANCE || p0.h < DRY_TOLERANCE) {
+        s_eta = 0.0; s_hu = 0.0; s_hv = 0.0;
+    } else {
+        s_eta = minmod(e_m1 - e_m2, e_p0 - e_m1);
+        s_hu = minmod(m1.hu - m2.hu, p0.hu - m1.hu);
+        s_hv = minmod(m1.hv - m2.hv, p0.hv - m1.hv);
+    }
+    const double eta_L = e_m1 + 0.5 * s_eta;
+    const double hu_L = m1.hu + 0.5 * s_hu;
+    const double hv_L = m1.hv + 0.5 * s_hv;
+    const double h_L = fmax0(eta_L - m1.b);
+
+    const double b_max = dmax(m1.b, p0.b);
+    const double h_hyd_l = fmax0(h_L + m1.b - b_max);
+    const double h_hyd_r = fmax0(h_R + p0.b - b_max);
+
+    double u_l, v_l, u_r, v_r;  // extract_velocity, rki_kernel_flux.F90:395-408
+    if (h_L > DRY_TOLERANCE) { u_l = hu_L / h_L; v_l = hv_L / h_L; } else { u_l = 0.0; v_l = 0.0; }
+    if (h_R > DRY_TOLERANCE) { u_r = hu_R / h_R; v_r = hv_R / h_R; } else { u_r = 0.0; v_r = 0.0; }
+
+    Face f;
+    if (YDIR)
+        riemann_y<HLLC>(g, h_hyd_l, h_hyd_l * u_l, h_hyd_l * v_l, h_hyd_r, h_hyd_r * u_r,
+                        h_hyd_r * v_r, f.fh, f.fhu, f.fhv);
+    else
+        riemann_x<HLLC>(g, h_hyd_l, h_hyd_l * u_l, h_hyd_l * v_l, h_hyd_r, h_hyd_r * u_r,
+                        h_hyd_r * v_r, f.fh, f.fhu, f.fhv);
+    f.a = h_hyd_r * h_hyd_r - h_R * h_R;
+    f.c = h_hyd_l * h_hyd_l;
+    f.d = h_L * h_L;
+    return f;
+}
+
+// Net flux divergence of one cell from its four faces,
+// rki_kernel_flux.F90:358-369.
+__device__ __forceinline__ void divergence(double g, double dx, double dy, const Face& xl,
+                                           const Face& xr, const Face& yb, const Face& yt,
+                                           double& fh, double& fhu, double& fhv) {
+    const double half_g = 0.5 * g;
+    fh = -(xr.fh - xl.fh) / dx - (yt.fh - yb.fh) / dy;
+    fhu = -(xr.fhu - xl.fhu) / dx - (yt.fhu - yb.fhu) / dy -
+          half_g * (xl.a - xr.c + xr.d) / dx;
+    fhv = -(xr.fhv - xl.fhv) / dx - (yt.fhv - yb.fhv) / dy -
+          half_g * (yb.a - yt.c + yt.d) / dy;
+}
+
+// Clamp + KP07 desingularization + Froude backstop,
+// rki_kernel_fused.F90:234-260 and :440-463.
+__device__ __forceinline__ void clamp_desing(double g, double& h, double& hu, double& hv) {
+    const double eps4 = 5.062499999999998e-16;  // DESING_EPS**4 folded (0x3CC23D5504170727)
+    if (h < 0.0) {
+        h = 0.0; hu = 0.0; hv = 0.0;
+    } else if (h < DRY_TOLERANCE) {
+        hu = 0.0; hv = 0.0;
+    } else {
+        const double h4 = pow4(h);
+        if (h4 < eps4) {
+            const double u = SQRT2 * h * hu / sqrt(h4 + eps4);
+            const double v = SQRT2 * h * hv / sqrt(h4 + eps4);
+            hu = h * u;
+            hv = h * v;
+        }
+        const double speed_sq = (hu * hu + hv * hv) / (h * h);
+        const double fr_max_sq = (FROUDE_BACKSTOP * FROUDE_BACKSTOP * g) * h;
+        if (speed_sq > fr_max_sq) {
+            const double fr_scale = sqrt(fr_max_sq / speed_sq);
+            hu = hu * fr_scale;
+            hv = hv * fr_scale;
+        }
+    }
+}
+
+struct SourceTerms {
+    double g;
+    double n_sq;        // manning_n**2
+    int has_friction;   // manning_n > 0
+    int has_coriolis;   // |f| > 1e-20
+    double cosf, sinf;  // cos(f dt), sin(f dt)
+};
+
+// Euler update + clamp + Manning + Coriolis, rki_kernel_fused.F90:228-290
+// (forcing inactive).
+__device__ __forceinline__ void stage_body(const SourceTerms& s, double dt, double fh, double fhu,
+                                           double fhv, double& h, double& hu, double& hv) {
+    h = h + dt * fh;
+    hu = hu + dt * fhu;
+    hv = hv + dt * fhv;
+    clamp_desing(s.g, h, hu, hv);
+    if (h > DRY_TOLERANCE) {
+        if (s.has_friction) {
+            const double u = hu / h;
+            const double v = hv / h;
+            const double speed = sqrt(u * u + v * v);
+            const double cf = s.g * s.n_sq / crm::pow13(h);
+            const double denom = 1.0 + dt * cf * speed / h;
+            hu = hu / denom;
+            hv = hv / denom;
+        }
+        if (s.has_coriolis) {
+            const double hu_old = hu, hv_old = hv;
+            hu = hu_old * s.cosf + hv_old * s.sinf;
+            hv = -hu_old * s.sinf + hv_old * s.cosf;
+        }
+    }
+}
+
+// Per-cell CFL speed term, rki_kernel_timestep.F90:56-69; 0 for dry cells.
+__device__ __forceinline__ double cfl_speed(double g, double dx, double dy, double h, double hu,
+                                            double hv) {
+    const double eps4 = 5.062499999999998e-16;
+    if (!(h > DRY_TOLERANCE)) return 0.0;
+    const double wave_speed = sqrt(g * h);
+    const double h4 = pow4(h);
+    const double denom = sqrt(h4 + eps4);
+    const double u_abs = fabs(hu) * SQRT2 * h / denom;
+    const double v_abs = fabs(hv) * SQRT2 * h / denom;
+    const double sx = (u_abs + wave_speed) / dx;
+    const double sy = (v_abs + wave_speed) / dy;
+    return dmax(sx, sy);
+}
+
+}  // namespace rk

@@ -1,0 +1,164 @@
+"""Host-side handle of one structured grid on the GPU.
+
+``Grid`` wraps the ``rkg_*`` C ABI: it creates device state, moves fields in
+and out, and advances the SSP-RK2 step with ``rki_step`` / ``rki_advance``
+semantics (src/api/rki_api.F90:358-410).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import BC, FIELDS, Config, check
+
+GRAVITY = 9.80665          # rki_constants.F90:38
+DEFAULT_TIDAL_PERIOD = 44712.0
+
+
+def make_config(nx: int, ny: int, dx: float, dy: float, *, g: float = GRAVITY,
+                cfl: float = 0.45, manning_n: float = 0.0, coriolis_f: float = 0.0,
+                dt_fixed: float = 0.0, t_end: float = 1.0, cfl_interval: int = 1,
+                use_hllc: bool = False,
+                bc: Sequence[str] = ("wall", "wall", "wall", "wall"),
+                tidal_amplitude: float = 0.0,
+                tidal_period: float = DEFAULT_TIDAL_PERIOD,
+                tidal_constituents: Optional[Iterable[tuple]] = None,
+                inflow_velocity: float = 0.0, discharge_flux: float = 0.0,
+                clamped_elevation: float = 0.0) -> Config:
+    L = _lib.load()
+    c = Config()
+    check("rkg_config_default", L.rkg_config_default(ctypes.byref(c)))
+    c.nx, c.ny, c.dx, c.dy = int(nx), int(ny), float(dx), float(dy)
+    c.g, c.cfl, c.manning_n, c.coriolis_f = float(g), float(cfl), float(manning_n), float(coriolis_f)
+    c.dt_fixed, c.t_end, c.cfl_interval = float(dt_fixed), float(t_end), int(cfl_interval)
+    c.use_hllc = 1 if use_hllc else 0
+    if len(bc) != 4:
+        raise ValueError("bc needs four entries (west, east, south, north)")
+    for s, name in enumerate(bc):
+        if name not in BC:
+            raise ValueError(f"unsupported boundary type {name!r}")
+        c.bc[s] = BC[name]
+    c.tidal_amplitude, c.tidal_period = float(tidal_amplitude), float(tidal_period)
+    if tidal_constituents:
+        tcs = list(tidal_constituents)
+        if len(tcs) > _lib.MAX_TC:
+            raise ValueError("at most 10 tidal constituents")
+        c.n_tidal_constituents = len(tcs)
+        for n, (amp, phase, omega) in enumerate(tcs):
+            c.tidal_amp[n], c.tidal_phase[n], c.tidal_omega[n] = amp, phase, omega
+    c.inflow_velocity = float(inflow_velocity)
+    c.discharge_flux = float(discharge_flux)
+    c.clamped_elevation = float(clamped_elevation)
+    return c
+
+
+class Grid:
+    """One structured grid resident in HBM (fields h, hu, hv, b)."""
+
+    def __init__(self, nx: int, ny: int, dx: float, dy: float, *, stream: Optional[int] = None,
+                 **kw):
+        self._lib = _lib.load()
+        self.config = make_config(nx, ny, dx, dy, **kw)
+        self.nx, self.ny = int(nx), int(ny)
+        h = ctypes.c_void_p()
+        check("rkg_create", self._lib.rkg_create(ctypes.byref(self.config),
+                                                  ctypes.c_void_p(stream) if stream else None,
+                                                  ctypes.byref(h)))
+        self._h = h
+
+    # -- lifecycle -----------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.rkg_destroy(ctypes.byref(self._h))
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if self._h is None or not self._h.value:
+            raise RuntimeError("grid is closed")
+        return self._h
+
+    # -- fields --------------------------------------------------------
+    def set_field(self, name: str, values) -> None:
+        a = np.asfortranarray(np.asarray(values, dtype=np.float64))
+        if a.shape != (self.nx, self.ny):
+            raise ValueError(f"{name}: expected shape {(self.nx, self.ny)}, got {a.shape}")
+        check("rkg_upload", self._lib.rkg_upload(self.handle, FIELDS[name],
+                                                 a.ctypes.data_as(ctypes.c_void_p), self.nx))
+
+    def get_field(self, name: str) -> np.ndarray:
+        out = np.empty((self.nx, self.ny), dtype=np.float64, order="F")
+        check("rkg_download", self._lib.rkg_download(self.handle, FIELDS[name],
+                                                     out.ctypes.data_as(ctypes.c_void_p), self.nx))
+        return out
+
+    def device_ptr(self, name: str):
+        p = ctypes.c_void_p()
+        pitch = ctypes.c_longlong()
+        check("rkg_field_device_ptr", self._lib.rkg_field_device_ptr(
+            self.handle, FIELDS[name], ctypes.byref(p), ctypes.byref(pitch)))
+        return p.value, pitch.value
+
+    # -- stepping ------------------------------------------------------
+    def step(self, n: int = 1) -> None:
+        check("rkg_step", self._lib.rkg_step(self.handle, int(n)))
+
+    def advance(self, target: float) -> int:
+        taken = ctypes.c_longlong()
+        check("rkg_advance", self._lib.rkg_advance(self.handle, float(target), ctypes.byref(taken)))
+        return taken.value
+
+    def sync(self) -> None:
+        check("rkg_sync", self._lib.rkg_sync(self.handle))
+
+    @property
+    def clock(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        check("rkg_get_clock", self._lib.rkg_get_clock(self.handle, ctypes.byref(t),
+                                                       ctypes.byref(dt), ctypes.byref(n)))
+        return t.value, dt.value, n.value
+
+    @property
+    def t(self) -> float:
+        return self.clock[0]
+
+    @property
+    def dt(self) -> float:
+        return self.clock[1]
+
+    @property
+    def step_count(self) -> int:
+        return self.clock[2]
+
+    def set_clock(self, t: float, step_count: int = 0) -> None:
+        check("rkg_set_clock", self._lib.rkg_set_clock(self.handle, float(t), int(step_count)))
+
+    def total_mass(self) -> float:
+        m = ctypes.c_double()
+        check("rkg_total_mass", self._lib.rkg_total_mass(self.handle, ctypes.byref(m)))
+        return m.value
+
+    @property
+    def tile_rows(self) -> int:
+        r = ctypes.c_int()
+        check("rkg_get_tile_rows", self._lib.rkg_get_tile_rows(self.handle, ctypes.byref(r)))
+        return r.value
+
+    @tile_rows.setter
+    def tile_rows(self, rows: int) -> None:
+        check("rkg_set_tile_rows", self._lib.rkg_set_tile_rows(self.handle, int(rows)))
