@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X structured shallow-water step.
+
+Workload: forward SSP-RK2 steps of a 4096 x 4096 FP64 tidal basin (dx = dy =
+20 m, mean depth 30 m with 64 seeded Gaussian bumps, west edge tidal
+eta = 1.0 sin(2 pi t / 44712 s), other edges walls, Manning n = 0.025, HLLC,
+CFL 0.45), state resident in HBM.  A step is one full SSP-RK2 step (two
+stage kernels) of the whole grid; the metric is cell-updates per second
+(interior cells x steps / wall time).
+
+Prints one JSON line with the throughput, the HBM roofline of the dominant
+stage kernel (kernel time from HIP events on the launching stream) and a
+CPU baseline (the oracle, OpenMP, on a bounded sample of the same run).
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "FP64 cell-updates/s, %HBM roofline @4096²; adjoint/fwd cost; batched vs loop"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per interior cell and launch (FP64 words x 8):
+# stage 1 reads h, hu, hv, b and writes U* (7 words); stage 2 reads U*, b and
+# U^n and writes U^{n+1} (10 words); 17 words = 136 B per cell-update.
+STAGE_BYTES = {1: 56, 2: 80}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--ny", type=int, default=4096)
+    ap.add_argument("--kernel-steps", type=int, default=50,
+                    help="steps of the per-kernel timing pass")
+    ap.add_argument("--tile-rows", type=int, default=0, help="0 = library default")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU work of the baseline sample (0 disables)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build's path is "
+              "single-GPU; multi-GPU sharding is described in DESIGN.md, not measured.",
+              file=sys.stderr)
+        return 2
+
+    from rakali_amd import Grid
+    from rakali_amd import scenarios as S
+
+    scn = S.tidal_bumps(nx=args.nx, ny=args.ny, steps=args.steps)
+    g = Grid(scn.nx, scn.ny, scn.dx, scn.dy, **scn.grid_kw)
+    if args.tile_rows:
+        g.tile_rows = args.tile_rows
+    for k in ("b", "h", "hu", "hv"):
+        g.set_field(k, getattr(scn, k))
+    cells = scn.nx * scn.ny
+
+    g.step(args.warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = cells * args.steps / wall
+
+    # per-kernel timing pass (events on the grid's stream around each launch)
+    g.profile(True)
+    g.step(args.kernel_steps)
+    ms1, ms2, n = g.profile_read()
+    g.profile(False)
+    avg = {1: ms1 / n, 2: ms2 / n}
+    dom = max(avg, key=avg.get)
+    bytes_launch = STAGE_BYTES[dom] * cells
+    achieved = bytes_launch / (avg[dom] * 1e-3) / 1e9
+    per_step_ms = avg[1] + avg[2]
+    step_gbs = (STAGE_BYTES[1] + STAGE_BYTES[2]) * cells / (per_step_ms * 1e-3) / 1e9
+    t_sim, dt, nsteps = g.clock
+    mass = g.total_mass()
+    g.close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "forward SSP-RK2, 4096x4096 tidal basin (west tide A=1 m, T=44712 s; "
+                        "walls elsewhere), dx=dy=20 m, 64 seeded bumps, Manning n=0.025, HLLC, CFL 0.45",
+            "nx": scn.nx, "ny": scn.ny, "dx": scn.dx,
+            "dry_fraction_initial": round(scn.dry_fraction(), 4),
+            "tile_rows": args.tile_rows or 32,
+            "simulated_time_s": t_sim, "last_dt_s": dt, "total_steps": nsteps, "mass": mass,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_stage<{dom}> (stage {dom})",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": avg[dom],
+            "stage_avg_ms": {"1": avg[1], "2": avg[2]},
+            "step_algorithmic_GBs": step_gbs,
+            "step_frac": step_gbs / HBM_PEAK_GBS,
+        },
+    }
+    if args.cpu_seconds > 0:
+        out["cpu_baseline"] = run_cpu_baseline(scn, args.cpu_seconds, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_cpu_baseline(scn, seconds: float, threads: int):
+    """The oracle (oracle/rk_oracle.c, OpenMP over rows) timed on this host on
+    a bounded sample of the same run: the same bed and initial state, as many
+    steps as fit in about `seconds` (at least one)."""
+    import oracle as O
+
+    nthreads = threads or min(16, os.cpu_count() or 1)
+    O.set_threads(nthreads)
+    kw = dict(scn.grid_kw)
+    bc = kw.pop("bc")
+    p = O.make_params(scn.nx, scn.ny, ng=2, dx=scn.dx, dy=scn.dy, bc=bc,
+                      use_hllc=int(kw.pop("use_hllc")), **kw)
+    sim = O.Sim(p)
+    for k in ("b", "h", "hu", "hv"):
+        sim.set_interior(k, getattr(scn, k))
+    sim.finalize()
+    t0 = time.perf_counter()
+    sim.step(1)
+    one = time.perf_counter() - t0
+    more = max(0, int(seconds / max(one, 1e-9)) - 1)
+    t0 = time.perf_counter()
+    if more:
+        sim.step(more)
+    rest = time.perf_counter() - t0
+    steps = 1 + more
+    wall = one + rest
+    return {
+        "value": scn.nx * scn.ny * steps / wall,
+        "unit": "cell-updates/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"{steps} SSP-RK2 steps of the same {scn.nx}x{scn.ny} run from the same initial "
+                  f"state ({wall:.1f} s of CPU work, {nthreads} OpenMP threads)",
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

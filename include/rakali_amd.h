@@ -128,6 +128,13 @@ int rkg_sync(rkg_grid *g);
 /* Rows per tile segment of the stage kernels (1..64; default 32). */
 int rkg_set_tile_rows(rkg_grid *g, int rows);
 int rkg_get_tile_rows(rkg_grid *g, int *rows);
+/* Kernel timing: when enabled, rkg_step brackets each stage launch with
+ * HIP events on the grid's stream and waits for every step, accumulating
+ * the stage-1 and stage-2 kernel times (ms).  Enabling resets the sums. */
+int rkg_profile(rkg_grid *g, int enable);
+int rkg_profile_read(rkg_grid *g, double *stage1_ms, double *stage2_ms, long long *steps);
+/* The hipStream_t the grid launches on. */
+int rkg_get_stream(rkg_grid *g, void **stream);
 /* Last-error text of this thread. */
 const char *rkg_last_error(void);
 

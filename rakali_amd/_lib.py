@@ -77,6 +77,9 @@ SIGNATURES = {
     "rkg_get_tile_rows": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     "rkg_last_error": (ctypes.c_char_p, []),
     "rkg_math_eval": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int]),
+    "rkg_profile": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_profile_read": (ctypes.c_int, [_P, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "rkg_get_stream": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
 }
 
 

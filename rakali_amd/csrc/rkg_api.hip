@@ -224,6 +224,10 @@ struct rkg_grid {
     int parity;
     int ty;
     bool speed_stale;
+    bool prof;             // time each stage launch with events
+    hipEvent_t ev[3];
+    double prof_ms[2];     // accumulated stage-1 / stage-2 kernel time
+    long long prof_n;
 };
 
 namespace {
@@ -296,14 +300,27 @@ int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
     GridDesc d = make_desc(g);
     const int ntiles = d.nstrips * d.nsegs;
     for (int k = 0; k < n; ++k) {
-        if (g->cfg.use_hllc) {
+        if (g->prof) HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+        if (g->cfg.use_hllc)
             hipLaunchKernelGGL((k_stage<1, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-            hipLaunchKernelGGL((k_stage<2, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-        } else {
+        else
             hipLaunchKernelGGL((k_stage<1, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        if (g->prof) HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+        if (g->cfg.use_hllc)
+            hipLaunchKernelGGL((k_stage<2, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        else
             hipLaunchKernelGGL((k_stage<2, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-        }
         g->parity ^= 1;
+        if (g->prof) {
+            float a = 0.f, b = 0.f;
+            HIP_TRY(hipEventRecord(g->ev[2], g->stream));
+            HIP_TRY(hipEventSynchronize(g->ev[2]));
+            HIP_TRY(hipEventElapsedTime(&a, g->ev[0], g->ev[1]));
+            HIP_TRY(hipEventElapsedTime(&b, g->ev[1], g->ev[2]));
+            g->prof_ms[0] += a;
+            g->prof_ms[1] += b;
+            g->prof_n += 1;
+        }
     }
     HIP_TRY(hipGetLastError());
     return RKG_OK;
@@ -408,6 +425,8 @@ void rkg_destroy(rkg_grid** pg) {
     if (g->mass_buf) (void)hipFree(g->mass_buf);
     if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
     if (g->mass_host) (void)hipHostFree(g->mass_host);
+    for (int k = 0; k < 3; ++k)
+        if (g->ev[k]) (void)hipEventDestroy(g->ev[k]);
     if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
     g->magic = GRID_DEAD;
     delete g;
@@ -531,6 +550,31 @@ int rkg_set_tile_rows(rkg_grid* g, int rows) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     if (rows < 1 || rows > WAVE) { set_error("tile rows must be in 1..64"); return RKG_ERR_RANGE; }
     g->ty = rows;
+    return RKG_OK;
+}
+
+int rkg_profile(rkg_grid* g, int enable) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    for (int k = 0; k < 3; ++k)
+        if (!g->ev[k]) HIP_TRY(hipEventCreate(&g->ev[k]));
+    g->prof = enable != 0;
+    g->prof_ms[0] = g->prof_ms[1] = 0.0;
+    g->prof_n = 0;
+    return RKG_OK;
+}
+
+int rkg_profile_read(rkg_grid* g, double* stage1_ms, double* stage2_ms, long long* steps) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (stage1_ms) *stage1_ms = g->prof_ms[0];
+    if (stage2_ms) *stage2_ms = g->prof_ms[1];
+    if (steps) *steps = g->prof_n;
+    return RKG_OK;
+}
+
+int rkg_get_stream(rkg_grid* g, void** stream) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!stream) return RKG_ERR_RANGE;
+    *stream = (void*)g->stream;
     return RKG_OK;
 }
 

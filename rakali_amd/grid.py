@@ -153,6 +153,17 @@ class Grid:
         check("rkg_total_mass", self._lib.rkg_total_mass(self.handle, ctypes.byref(m)))
         return m.value
 
+    def profile(self, enable: bool = True) -> None:
+        """Time every stage launch (synchronizes each step); resets the sums."""
+        check("rkg_profile", self._lib.rkg_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self):
+        """(stage-1 ms, stage-2 ms, steps) accumulated since profile()."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        check("rkg_profile_read", self._lib.rkg_profile_read(self.handle, ctypes.byref(a),
+                                                             ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
+
     @property
     def tile_rows(self) -> int:
         r = ctypes.c_int()
