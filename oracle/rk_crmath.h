@@ -13,9 +13,10 @@
  * about 1e-26 relative.
  *
  * Methods (own implementation):
- *   cbrt seed by exponent splitting and a quadratic fit, four Newton steps,
- *   one double-double residual correction; the exponent 1.0/3.0 is the
- *   double 1/3 - 2^-54/3, applied as the factor exp(-(2^-54/3) ln x).
+ *   inverse cube root seeded by exponent splitting and a quadratic fit,
+ *   four division-free Newton steps, one double-double residual
+ *   correction; the exponent 1.0/3.0 is the double 1/3 - 2^-54/3, applied
+ *   as the factor exp(-(2^-54/3) ln x).
  *   sin/cos by Cody-Waite reduction with pi/2 in three doubles and Taylor
  *   series (through x^25 / x^26) in double-double.
  * TEST INFRASTRUCTURE: part of the oracle.
@@ -93,19 +94,20 @@ static inline double crm_pow13(double x)
     int q = (e >= 0) ? e / 3 : -((-e + 2) / 3); /* floor(e / 3) */
     int r = e - 3 * q;                          /* 0, 1, 2 */
     double y = m * (double)(1 << r);            /* [1, 8), exact */
-    /* seed: quadratic fit of cbrt(m) on [1, 2] (relative error < 1.2e-3)
-     * times cbrt(2^r); four Newton steps then converge to rounding */
-    double c = 0.62490765 + m * (0.43556508 + m * (-0.05930307));
-    c = c * (r == 0 ? 1.0 : (r == 1 ? 1.2599210498948732 : 1.5874010519681994));
-    for (int it = 0; it < 4; ++it) c = (2.0 * c + y / (c * c)) / 3.0;
-    double r0 = c * crm_bits2d((uint64_t)(q + 1023) << 52); /* exact scaling */
-    /* residual x - r0^3 in double-double */
+    /* y^(-1/3): quadratic fit of m^(-1/3) on [1, 2] (relative error
+     * < 3.3e-3) times 2^(-r/3), then four division-free Newton steps */
+    double rr = 1.37911983 + m * (-0.47289307 + m * 0.09056511);
+    rr = rr * (r == 0 ? 1.0 : (r == 1 ? 0.7937005259840998 : 0.6299605249474366));
+    for (int it = 0; it < 4; ++it) rr = rr + (rr * (1.0 - y * ((rr * rr) * rr))) * (1.0 / 3.0);
+    double r0 = ((y * rr) * rr) * crm_bits2d((uint64_t)(q + 1023) << 52); /* ~x^(1/3) */
+    /* residual x - r0^3 in double-double, Newton correction res/(3 r0^2) */
     double p = r0 * r0;
     double pl = fma(r0, r0, -p);
     double t3 = p * r0;
     double t3l = fma(p, r0, -t3);
     double res = (x - t3) - (t3l + pl * r0);
-    double corr = res / (3.0 * p);
+    double inv = ((rr * rr) * (1.0 / 3.0)) * crm_bits2d((uint64_t)(1023 - 2 * q) << 52);
+    double corr = res * inv;
     /* exponent is 1/3 - 2^-54/3: multiply by exp(-(2^-54/3) ln x) */
     double t = -(crm_log_approx(x) * (1.0 / 3.0)) * 5.551115123125783e-17;
     return r0 + (corr + r0 * t);

@@ -7,10 +7,13 @@ every call raises -- there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "librakali_amd.so"
+# RAKALI_AMD_LIB overrides the in-tree library (as RAKALI_LIB does for the
+# reference's librakali, python/rakali/_core.py:28-44).
+LIB_PATH = Path(os.environ.get("RAKALI_AMD_LIB") or (_PKG / "librakali_amd.so"))
 
 MAX_TC = 10
 

@@ -12,9 +12,10 @@
 // the same bits on every IEEE machine and the correctly rounded result
 // except within ~1e-26 relative of a rounding midpoint.
 //
-// pow13: cube root seeded by exponent splitting and a quadratic fit, four
-// Newton steps, one double-double residual correction, then the factor
-// exp(-(2^-54/3) ln x) because the double 1.0/3.0 is 1/3 - 2^-54/3.
+// pow13: inverse cube root seeded by exponent splitting and a quadratic
+// fit, four division-free Newton steps, one double-double residual
+// correction, then the factor exp(-(2^-54/3) ln x) because the double
+// 1.0/3.0 is 1/3 - 2^-54/3.
 // sin/cos: Cody-Waite reduction by pi/2 held in three doubles, Taylor
 // series to x^25 / x^26 in double-double.
 #pragma once
@@ -86,16 +87,18 @@ __host__ __device__ __forceinline__ double pow13(double x) {
     const int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);  // floor(e / 3)
     const int r = e - 3 * q;
     const double y = m * (double)(1 << r);  // [1, 8)
-    double c = 0.62490765 + m * (0.43556508 + m * (-0.05930307));
-    c = c * (r == 0 ? 1.0 : (r == 1 ? 1.2599210498948732 : 1.5874010519681994));
-    for (int it = 0; it < 4; ++it) c = (2.0 * c + y / (c * c)) / 3.0;
-    const double r0 = c * bits2d((uint64_t)(q + 1023) << 52);
+    // y^(-1/3): quadratic seed (relative error < 3.3e-3), four Newton steps
+    double rr = 1.37911983 + m * (-0.47289307 + m * 0.09056511);
+    rr = rr * (r == 0 ? 1.0 : (r == 1 ? 0.7937005259840998 : 0.6299605249474366));
+    for (int it = 0; it < 4; ++it) rr = rr + (rr * (1.0 - y * ((rr * rr) * rr))) * (1.0 / 3.0);
+    const double r0 = ((y * rr) * rr) * bits2d((uint64_t)(q + 1023) << 52);
     const double p = r0 * r0;
     const double pl = fma(r0, r0, -p);
     const double t3 = p * r0;
     const double t3l = fma(p, r0, -t3);
     const double res = (x - t3) - (t3l + pl * r0);
-    const double corr = res / (3.0 * p);
+    const double inv = ((rr * rr) * (1.0 / 3.0)) * bits2d((uint64_t)(1023 - 2 * q) << 52);
+    const double corr = res * inv;
     const double t = -(log_approx(x) * (1.0 / 3.0)) * 5.551115123125783e-17;  // 2^-54
     return r0 + (corr + r0 * t);
 }

@@ -24,6 +24,10 @@
 
 using namespace rk;
 
+#ifndef RKG_WAVES_PER_EU
+#define RKG_WAVES_PER_EU 3
+#endif
+
 namespace {
 
 constexpr unsigned GRID_MAGIC = 0x72616B47u;  // 'rakG'
@@ -91,7 +95,7 @@ __device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, i
 }
 
 template <int STAGE, bool HLLC>
-__global__ __launch_bounds__(WAVE) void k_stage(GridDesc d, int p, int mode, double tlim) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage(GridDesc d, int p, int mode, double tlim) {
     __shared__ TileLds L;
     StageCtx s;
     if (STAGE == 1) {
@@ -121,6 +125,8 @@ __global__ __launch_bounds__(WAVE) void k_stage(GridDesc d, int p, int mode, dou
         s.eta_tidal = tidal_eta(d, cq.t);  // bc%time = t + dt
     }
     s.src = make_src(d, s.dt);
+    s.rdx = recip(d.dx);
+    s.rdy = recip(d.dy);
 
     const int ntiles = d.nstrips * d.nsegs;
     const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -153,11 +159,12 @@ __global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long lo
 // rki_kernel_timestep.F90:53-70).
 __global__ __launch_bounds__(256) void k_init_speed(GridDesc d, int p) {
     double m = 0.0;
+    const Recip rdx = recip(d.dx), rdy = recip(d.dy);
     const long long n = (long long)d.nx * d.ny;
     for (long long k = blockIdx.x * 256ll + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
         const int j = (int)(k / d.nx), i = (int)(k % d.nx);
         const long long o = (long long)j * d.pitch + i;
-        m = dmax(m, cfl_speed(d.g, d.dx, d.dy, d.ah[o], d.ahu[o], d.ahv[o]));
+        m = dmax(m, cfl_speed<true>(d.g, rdx, rdy, d.ah[o], d.ahu[o], d.ahv[o]));
     }
     m = wave_max(m);
     if ((threadIdx.x & (WAVE - 1)) == 0 && m > 0.0)
@@ -335,7 +342,10 @@ int read_clock(rkg_grid* g, CtrlSlot* out) {
 
 int check_config(const rkg_config* c) {
     if (c->nx < 2 || c->ny < 2) { set_error("nx and ny must be >= 2 (got %d, %d)", c->nx, c->ny); return RKG_ERR_RANGE; }
-    if (!(c->dx > 0.0) || !(c->dy > 0.0)) { set_error("dx and dy must be positive"); return RKG_ERR_RANGE; }
+    if (!(c->dx >= 1e-6 && c->dx <= 1e9) || !(c->dy >= 1e-6 && c->dy <= 1e9)) {
+        set_error("dx and dy must lie in [1e-6, 1e9]");
+        return RKG_ERR_RANGE;
+    }
     if (c->cfl_interval < 1) { set_error("cfl_interval must be >= 1"); return RKG_ERR_RANGE; }
     if (c->n_tidal_constituents < 0 || c->n_tidal_constituents > RKG_MAX_TIDAL_CONSTITUENTS) {
         set_error("n_tidal_constituents out of range"); return RKG_ERR_RANGE;

@@ -12,8 +12,9 @@
 // value is the IEEE result of the same operation sequence.  The reference
 // evaluates every interior face twice (once from each cell,
 // rki_kernel_flux.F90:371-372) and both evaluations are bit-identical,
-// because a face depends only on the four cells across it; `face()` below
-// evaluates each face once and hands it to both cells.
+// because a face depends only on the four cells across it; here each
+// cell's slopes are evaluated once (cell_states) and each face once
+// (face_flux), and the face is handed to both of its cells.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,6 +54,45 @@ __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : 
 // x**4 as LLVM flang lowers it for a run-time base: ((x*x)*x)*x.
 __device__ __forceinline__ double pow4(double x) { return ((x * x) * x) * x; }
 
+// IEEE division by a denominator shared between several numerators.
+// The compiler's f64 division is v_div_scale (identity for operands away
+// from the exponent extremes), v_rcp, two Newton refinements of the
+// reciprocal, a product, a residual fma and the correction fma
+// (v_div_fmas), then v_div_fixup (identity for finite, non-zero operands).
+// Recip keeps the refined reciprocal so each further quotient by the same
+// denominator costs a product and two fmas and returns the same bits as
+// `n / d`.  The residual is formed as -fma(d, q0, -n) so that zero
+// numerators keep their sign.  The unscaled sequence equals the IEEE
+// quotient while |n| >= 2^-969 (or n = 0) and the denominator lies in
+// [1e-8, 1e12]; callers guarantee the numerator bound by running a row
+// with EXACT = true (plain division) whenever a cell of its stencil holds
+// a non-zero h, hu, hv or b below 2^-300 in magnitude (see tiny()).
+struct Recip {
+    double d, y;
+};
+
+__device__ __forceinline__ Recip recip(double d) {
+    const double y0 = __builtin_amdgcn_rcp(d);
+    const double y1 = fma(y0, fma(-d, y0, 1.0), y0);
+    const double y2 = fma(y1, fma(-d, y1, 1.0), y1);
+    return Recip{d, y2};
+}
+
+template <bool EXACT>
+__device__ __forceinline__ double rdiv(double n, const Recip& r) {
+    if (EXACT) return n / r.d;
+    const double q0 = n * r.y;
+    const double t = fma(r.d, q0, -n);
+    return fma(-t, r.y, q0);
+}
+
+// Non-zero and below 2^-300 in magnitude: a value for which some quotient
+// of the step could leave the range where rdiv<false> is exact.
+__device__ __forceinline__ bool tiny(double x) { return fabs(x) < 0x1p-300 && x != 0.0; }
+__device__ __forceinline__ bool tiny(const Cell& c) {
+    return tiny(c.h) || tiny(c.hu) || tiny(c.hv) || tiny(c.b);
+}
+
 // minmod, rki_kernel_flux.F90:380-393
 __device__ __forceinline__ double minmod(double a, double b) {
     if (a * b <= 0.0) return 0.0;
@@ -60,13 +100,13 @@ __device__ __forceinline__ double minmod(double a, double b) {
 }
 
 // HLL / HLLC normal flux in x: rki_kernel_flux.F90:410-495 and 584-684.
-template <bool HLLC>
+template <bool HLLC, bool EXACT>
 __device__ __forceinline__ void riemann_x(double g, double h_l, double hu_l, double hv_l,
                                           double h_r, double hu_r, double hv_r,
                                           double& f_h, double& f_hu, double& f_hv) {
-    double u_l, v_l, u_r, v_r;
-    if (h_l > DRY_TOLERANCE) { u_l = hu_l / h_l; v_l = hv_l / h_l; } else { u_l = 0.0; v_l = 0.0; }
-    if (h_r > DRY_TOLERANCE) { u_r = hu_r / h_r; v_r = hv_r / h_r; } else { u_r = 0.0; v_r = 0.0; }
+    double u_l = 0.0, v_l = 0.0, u_r = 0.0, v_r = 0.0;
+    if (h_l > DRY_TOLERANCE) { const Recip r = recip(h_l); u_l = rdiv<EXACT>(hu_l, r); v_l = rdiv<EXACT>(hv_l, r); }
+    if (h_r > DRY_TOLERANCE) { const Recip r = recip(h_r); u_r = rdiv<EXACT>(hu_r, r); v_r = rdiv<EXACT>(hv_r, r); }
     const double a_l = sqrt(g * fmax0(h_l));
     const double a_r = sqrt(g * fmax0(h_r));
     double s_l, s_r;
@@ -107,13 +147,13 @@ __device__ __forceinline__ void riemann_x(double g, double h_l, double hu_l, dou
 
 // HLL / HLLC normal flux in y: rki_kernel_flux.F90:497-582 and 686-783.
 // Arguments are (h, hu, hv) of the bottom then the top state, as there.
-template <bool HLLC>
+template <bool HLLC, bool EXACT>
 __device__ __forceinline__ void riemann_y(double g, double h_b, double hu_b, double hv_b,
                                           double h_t, double hu_t, double hv_t,
                                           double& g_h, double& g_hu, double& g_hv) {
-    double u_b, v_b, u_t, v_t;
-    if (h_b > DRY_TOLERANCE) { u_b = hu_b / h_b; v_b = hv_b / h_b; } else { u_b = 0.0; v_b = 0.0; }
-    if (h_t > DRY_TOLERANCE) { u_t = hu_t / h_t; v_t = hv_t / h_t; } else { u_t = 0.0; v_t = 0.0; }
+    double u_b = 0.0, v_b = 0.0, u_t = 0.0, v_t = 0.0;
+    if (h_b > DRY_TOLERANCE) { const Recip r = recip(h_b); u_b = rdiv<EXACT>(hu_b, r); v_b = rdiv<EXACT>(hv_b, r); }
+    if (h_t > DRY_TOLERANCE) { const Recip r = recip(h_t); u_t = rdiv<EXACT>(hu_t, r); v_t = rdiv<EXACT>(hv_t, r); }
     const double a_b = sqrt(g * fmax0(h_b));
     const double a_t = sqrt(g * fmax0(h_t));
     double s_l, s_r;
@@ -151,82 +191,83 @@ __device__ __forceinline__ void riemann_y(double g, double h_b, double hu_b, dou
     }
 }
 
-// One face between cell L = m1 and cell R = p0, from the four cells
-// m2, m1, p0, p1 along the face normal (YDIR selects the y-face Riemann
-// solver; the reconstruction is the same in both directions).
-// rki_kernel_flux.F90:158-224 (x left face) restated face-centrically:
-// the left cell's slope uses (m2, m1, p0) and the right cell's (m1, p0, p1).
-template <bool HLLC, bool YDIR>
-__device__ __forceinline__ Face face(double g, const Cell& m2, const Cell& m1, const Cell& p0,
-                                     const Cell& p1) {
-    const double e_m2 = m2.h + m2.b;
-    const double e_m1 = m1.h + m1.b;
-    const double e_p0 = p0.h + p0.b;
-    const double e_p1 = p1.h + p1.b;
+// Reconstructed state of a cell at one of its faces along a direction:
+// h = max(0, eta_face - b) and the reconstructed momenta (the bed is the
+// cell's own).
+struct FState {
+    double h, hu, hv;
+};
 
-    // right cell (p0) slope toward its left face: eta_R = eta_c - 0.5*seta
+// Minmod-limited MUSCL states of cell c at its low (lo) and high (hi) face
+// along one direction, from its neighbours m (below) and p (above):
+// rki_kernel_flux.F90:158-181 for the slope (zeroed near wet/dry fronts),
+// :186-189 / :229-232 for the face states.  Each face of the reference
+// re-evaluates this slope for both of its cells; a cell's slope is the
+// same from both of its faces, so it is evaluated once here.
+__device__ __forceinline__ void cell_states(const Cell& m, const Cell& c, const Cell& p, FState& lo,
+                                            FState& hi) {
+    const double e_m = m.h + m.b;
+    const double e_c = c.h + c.b;
+    const double e_p = p.h + p.b;
     double s_eta, s_hu, s_hv;
-    if (p0.h < THIN_LAYER_THRESHOLD || m1.h < DRY_TOLERANCE || p1.h < DRY_TOLERANCE) {
+    if (c.h < THIN_LAYER_THRESHOLD || m.h < DRY_TOLERANCE || p.h < DRY_TOLERANCE) {
         s_eta = 0.0; s_hu = 0.0; s_hv = 0.0;
     } else {
-        s_eta = minmod(e_p0 - e_m1, e_p1 - e_p0);
-        s_hu = minmod(p0.hu - m1.hu, p1.hu - p0.hu);
-        s_hv = minmod(p0.hv - m1.hv, p1.hv - p0.hv);
+        s_eta = minmod(e_c - e_m, e_p - e_c);
+        s_hu = minmod(c.hu - m.hu, p.hu - c.hu);
+        s_hv = minmod(c.hv - m.hv, p.hv - c.hv);
     }
-    const double eta_R = e_p0 - 0.5 * s_eta;
-    const double hu_R = p0.hu - 0.5 * s_hu;
-    const double hv_R = p0.hv - 0.5 * s_hv;
-    const double h_R = fmax0(eta_R - p0.b);
+    lo.h = fmax0((e_c - 0.5 * s_eta) - c.b);
+    lo.hu = c.hu - 0.5 * s_hu;
+    lo.hv = c.hv - 0.5 * s_hv;
+    hi.h = fmax0((e_c + 0.5 * s_eta) - c.b);
+    hi.hu = c.hu + 0.5 * s_hu;
+    hi.hv = c.hv + 0.5 * s_hv;
+}
 
-    // left cell (m1) slope toward its right face
-    if (m1.h < THIN_LAYER_THRESHOLD || m2.h < DRY_TOLERANCE || p0.h < DRY_TOLERANCE) {
-        s_eta = 0.0; s_hu = 0.0; s_hv = 0.0;
-    } else {
-        s_eta = minmod(e_m1 - e_m2, e_p0 - e_m1);
-        s_hu = minmod(m1.hu - m2.hu, p0.hu - m1.hu);
-        s_hv = minmod(m1.hv - m2.hv, p0.hv - m1.hv);
-    }
-    const double eta_L = e_m1 + 0.5 * s_eta;
-    const double hu_L = m1.hu + 0.5 * s_hu;
-    const double hv_L = m1.hv + 0.5 * s_hv;
-    const double h_L = fmax0(eta_L - m1.b);
-
-    const double b_max = dmax(m1.b, p0.b);
-    const double h_hyd_l = fmax0(h_L + m1.b - b_max);
-    const double h_hyd_r = fmax0(h_R + p0.b - b_max);
-
-    double u_l, v_l, u_r, v_r;  // extract_velocity, rki_kernel_flux.F90:395-408
-    if (h_L > DRY_TOLERANCE) { u_l = hu_L / h_L; v_l = hv_L / h_L; } else { u_l = 0.0; v_l = 0.0; }
-    if (h_R > DRY_TOLERANCE) { u_r = hu_R / h_R; v_r = hv_R / h_R; } else { u_r = 0.0; v_r = 0.0; }
-
+// One face between the high state L of its left cell (bed bL) and the low
+// state R of its right cell (bed bR): Audusse hydrostatic reconstruction
+// (rki_kernel_flux.F90:208-210), velocities (extract_velocity, :395-408)
+// and the HLL/HLLC flux of the h_hyd-weighted momenta (:216-224).
+template <bool HLLC, bool YDIR, bool EXACT>
+__device__ __forceinline__ Face face_flux(double g, const FState& L, double bL, const FState& R,
+                                          double bR) {
+    const double b_max = dmax(bL, bR);
+    const double h_hyd_l = fmax0(L.h + bL - b_max);
+    const double h_hyd_r = fmax0(R.h + bR - b_max);
+    double u_l = 0.0, v_l = 0.0, u_r = 0.0, v_r = 0.0;
+    if (L.h > DRY_TOLERANCE) { const Recip r = recip(L.h); u_l = rdiv<EXACT>(L.hu, r); v_l = rdiv<EXACT>(L.hv, r); }
+    if (R.h > DRY_TOLERANCE) { const Recip r = recip(R.h); u_r = rdiv<EXACT>(R.hu, r); v_r = rdiv<EXACT>(R.hv, r); }
     Face f;
     if (YDIR)
-        riemann_y<HLLC>(g, h_hyd_l, h_hyd_l * u_l, h_hyd_l * v_l, h_hyd_r, h_hyd_r * u_r,
+        riemann_y<HLLC, EXACT>(g, h_hyd_l, h_hyd_l * u_l, h_hyd_l * v_l, h_hyd_r, h_hyd_r * u_r,
                         h_hyd_r * v_r, f.fh, f.fhu, f.fhv);
     else
-        riemann_x<HLLC>(g, h_hyd_l, h_hyd_l * u_l, h_hyd_l * v_l, h_hyd_r, h_hyd_r * u_r,
+        riemann_x<HLLC, EXACT>(g, h_hyd_l, h_hyd_l * u_l, h_hyd_l * v_l, h_hyd_r, h_hyd_r * u_r,
                         h_hyd_r * v_r, f.fh, f.fhu, f.fhv);
-    f.a = h_hyd_r * h_hyd_r - h_R * h_R;
+    f.a = h_hyd_r * h_hyd_r - R.h * R.h;
     f.c = h_hyd_l * h_hyd_l;
-    f.d = h_L * h_L;
+    f.d = L.h * L.h;
     return f;
 }
 
 // Net flux divergence of one cell from its four faces,
-// rki_kernel_flux.F90:358-369.
-__device__ __forceinline__ void divergence(double g, double dx, double dy, const Face& xl,
-                                           const Face& xr, const Face& yb, const Face& yt,
-                                           double& fh, double& fhu, double& fhv) {
+// rki_kernel_flux.F90:358-369 (rdx, rdy: reciprocals of dx, dy).
+template <bool EXACT>
+__device__ __forceinline__ void divergence(double g, const Recip& rdx, const Recip& rdy,
+                                           const Face& xl, const Face& xr, const Face& yb,
+                                           const Face& yt, double& fh, double& fhu, double& fhv) {
     const double half_g = 0.5 * g;
-    fh = -(xr.fh - xl.fh) / dx - (yt.fh - yb.fh) / dy;
-    fhu = -(xr.fhu - xl.fhu) / dx - (yt.fhu - yb.fhu) / dy -
-          half_g * (xl.a - xr.c + xr.d) / dx;
-    fhv = -(xr.fhv - xl.fhv) / dx - (yt.fhv - yb.fhv) / dy -
-          half_g * (yb.a - yt.c + yt.d) / dy;
+    fh = rdiv<EXACT>(-(xr.fh - xl.fh), rdx) - rdiv<EXACT>(yt.fh - yb.fh, rdy);
+    fhu = rdiv<EXACT>(-(xr.fhu - xl.fhu), rdx) - rdiv<EXACT>(yt.fhu - yb.fhu, rdy) -
+          rdiv<EXACT>(half_g * (xl.a - xr.c + xr.d), rdx);
+    fhv = rdiv<EXACT>(-(xr.fhv - xl.fhv), rdx) - rdiv<EXACT>(yt.fhv - yb.fhv, rdy) -
+          rdiv<EXACT>(half_g * (yb.a - yt.c + yt.d), rdy);
 }
 
 // Clamp + KP07 desingularization + Froude backstop,
 // rki_kernel_fused.F90:234-260 and :440-463.
+template <bool EXACT>
 __device__ __forceinline__ void clamp_desing(double g, double& h, double& hu, double& hv) {
     const double eps4 = 5.062499999999998e-16;  // DESING_EPS**4 folded (0x3CC23D5504170727)
     if (h < 0.0) {
@@ -236,8 +277,9 @@ __device__ __forceinline__ void clamp_desing(double g, double& h, double& hu, do
     } else {
         const double h4 = pow4(h);
         if (h4 < eps4) {
-            const double u = SQRT2 * h * hu / sqrt(h4 + eps4);
-            const double v = SQRT2 * h * hv / sqrt(h4 + eps4);
+            const Recip r = recip(sqrt(h4 + eps4));
+            const double u = rdiv<EXACT>(SQRT2 * h * hu, r);
+            const double v = rdiv<EXACT>(SQRT2 * h * hv, r);
             hu = h * u;
             hv = h * v;
         }
@@ -261,21 +303,23 @@ struct SourceTerms {
 
 // Euler update + clamp + Manning + Coriolis, rki_kernel_fused.F90:228-290
 // (forcing inactive).
+template <bool EXACT>
 __device__ __forceinline__ void stage_body(const SourceTerms& s, double dt, double fh, double fhu,
                                            double fhv, double& h, double& hu, double& hv) {
     h = h + dt * fh;
     hu = hu + dt * fhu;
     hv = hv + dt * fhv;
-    clamp_desing(s.g, h, hu, hv);
+    clamp_desing<EXACT>(s.g, h, hu, hv);
     if (h > DRY_TOLERANCE) {
         if (s.has_friction) {
-            const double u = hu / h;
-            const double v = hv / h;
+            const Recip rh = recip(h);
+            const double u = rdiv<EXACT>(hu, rh);
+            const double v = rdiv<EXACT>(hv, rh);
             const double speed = sqrt(u * u + v * v);
             const double cf = s.g * s.n_sq / crm::pow13(h);
-            const double denom = 1.0 + dt * cf * speed / h;
-            hu = hu / denom;
-            hv = hv / denom;
+            const Recip rd = recip(1.0 + rdiv<EXACT>(dt * cf * speed, rh));
+            hu = rdiv<EXACT>(hu, rd);
+            hv = rdiv<EXACT>(hv, rd);
         }
         if (s.has_coriolis) {
             const double hu_old = hu, hv_old = hv;
@@ -286,17 +330,18 @@ __device__ __forceinline__ void stage_body(const SourceTerms& s, double dt, doub
 }
 
 // Per-cell CFL speed term, rki_kernel_timestep.F90:56-69; 0 for dry cells.
-__device__ __forceinline__ double cfl_speed(double g, double dx, double dy, double h, double hu,
-                                            double hv) {
+template <bool EXACT>
+__device__ __forceinline__ double cfl_speed(double g, const Recip& rdx, const Recip& rdy, double h,
+                                            double hu, double hv) {
     const double eps4 = 5.062499999999998e-16;
     if (!(h > DRY_TOLERANCE)) return 0.0;
     const double wave_speed = sqrt(g * h);
     const double h4 = pow4(h);
-    const double denom = sqrt(h4 + eps4);
-    const double u_abs = fabs(hu) * SQRT2 * h / denom;
-    const double v_abs = fabs(hv) * SQRT2 * h / denom;
-    const double sx = (u_abs + wave_speed) / dx;
-    const double sy = (v_abs + wave_speed) / dy;
+    const Recip rd = recip(sqrt(h4 + eps4));
+    const double u_abs = rdiv<EXACT>(fabs(hu) * SQRT2 * h, rd);
+    const double v_abs = rdiv<EXACT>(fabs(hv) * SQRT2 * h, rd);
+    const double sx = rdiv<EXACT>(u_abs + wave_speed, rdx);
+    const double sy = rdiv<EXACT>(v_abs + wave_speed, rdy);
     return dmax(sx, sy);
 }
 

@@ -175,12 +175,22 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 }
 
 __device__ __forceinline__ double shfl_down1(double v) { return __shfl_down(v, 1, WAVE); }
+__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, WAVE); }
 
 __device__ __forceinline__ Face shfl_down1(const Face& f) {
     Face r;
     r.fh = shfl_down1(f.fh); r.fhu = shfl_down1(f.fhu); r.fhv = shfl_down1(f.fhv);
     r.a = shfl_down1(f.a); r.c = shfl_down1(f.c); r.d = shfl_down1(f.d);
     return r;
+}
+__device__ __forceinline__ Cell shfl_up1(const Cell& x) {
+    return Cell{shfl_up1(x.h), shfl_up1(x.hu), shfl_up1(x.hv), shfl_up1(x.b)};
+}
+__device__ __forceinline__ Cell shfl_down1(const Cell& x) {
+    return Cell{shfl_down1(x.h), shfl_down1(x.hu), shfl_down1(x.hv), shfl_down1(x.b)};
+}
+__device__ __forceinline__ FState shfl_up1(const FState& x) {
+    return FState{shfl_up1(x.h), shfl_up1(x.hu), shfl_up1(x.hv)};
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -189,9 +199,14 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Per-segment data of the two edge columns of a strip, one row per entry:
+// raw cells c0-1 and c0+64, the high x-state of c0-1, and the x-face
+// between c0+63 and c0+64.
 struct TileLds {
-    double rh[WAVE + 3], rhu[WAVE + 3], rhv[WAVE + 3], rb[WAVE + 3];
-    double bnd[6][WAVE];
+    double hl[4][WAVE];
+    double hr[4][WAVE];
+    double ph[3][WAVE];
+    double pf[6][WAVE];
 };
 
 struct StageCtx {
@@ -199,16 +214,32 @@ struct StageCtx {
     double* oh; double* ohu; double* ohv;                     // stage output
     double dt;
     double eta_tidal;
+    Recip rdx, rdy;
     SourceTerms src;
 };
 
+template <bool EDGE>
+__device__ __forceinline__ Cell ld(const GridDesc& d, const StageCtx& s, int c, int j) {
+    return load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j, s.eta_tidal);
+}
+
+template <bool B>
+struct ExactTag {
+    static constexpr bool value = B;
+};
+
 // One 64-column x d.ty-row tile of one RK stage, processed by one
-// wavefront marching down the rows.  Lane l owns column c = c0 + l.
-// Per row it evaluates the x-face on its left (columns c-2..c+1 read from
-// an LDS copy of the row) and the y-face above it (rows j-1..j+2 held in a
-// register window); the face below is carried from the previous row and
-// the face right of lane 63 comes from a batch evaluated up front, so every
-// face of the tile is evaluated exactly once.
+// wavefront marching down the rows; lane l owns column c = c0 + l.
+// Row j: the y-slope of row j+1 is formed from the raw rows j, j+1, j+2
+// held in registers, giving the face above row j together with the top
+// state of row j carried from the previous row; the x-slope of row j takes
+// the neighbouring columns by wave shuffles, the face left of each cell is
+// evaluated by its lane and the face right of it is the next lane's.  The
+// edge columns come from a per-segment batch (one row per lane) held in
+// LDS.  Every slope and every face of the tile is evaluated once.
+// A row whose stencil holds a tiny non-zero value (see tiny()) anywhere in
+// the wavefront is evaluated with plain divisions; all others with the
+// shared-reciprocal ones; both give the IEEE quotients.
 // STAGE 1 writes U* to B; STAGE 2 reads U* and U^n, applies the RK2 average
 // and writes U^{n+1} over A, returning the lane's max CFL speed.
 template <int STAGE, bool HLLC, bool EDGE>
@@ -224,34 +255,59 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     const long long pitch = d.pitch;
     double smax = 0.0;
 
-    // (1) faces right of column c0+63, one row per lane.
-    if (c0 + WAVE - 1 < d.nx && lane < nrows) {
+    // (1) edge columns of the segment, one row per lane
+    bool edge_tiny = false;
+    if (lane < nrows) {
         const int j = j0 + lane;
-        const Cell m2 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c0 + 62, j, s.eta_tidal);
-        const Cell m1 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c0 + 63, j, s.eta_tidal);
-        const Cell p0 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c0 + 64, j, s.eta_tidal);
-        const Cell p1 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c0 + 65, j, s.eta_tidal);
-        const Face f = face<HLLC, false>(g, m2, m1, p0, p1);
-        L.bnd[0][lane] = f.fh; L.bnd[1][lane] = f.fhu; L.bnd[2][lane] = f.fhv;
-        L.bnd[3][lane] = f.a; L.bnd[4][lane] = f.c; L.bnd[5][lane] = f.d;
+        const Cell a = ld<EDGE>(d, s, c0 - 2, j);
+        const Cell bl = ld<EDGE>(d, s, c0 - 1, j);
+        const Cell cc = ld<EDGE>(d, s, c0, j);
+        FState lo, hi;
+        cell_states(a, bl, cc, lo, hi);
+        L.hl[0][lane] = bl.h; L.hl[1][lane] = bl.hu; L.hl[2][lane] = bl.hv; L.hl[3][lane] = bl.b;
+        L.ph[0][lane] = hi.h; L.ph[1][lane] = hi.hu; L.ph[2][lane] = hi.hv;
+        const Cell m1 = ld<EDGE>(d, s, c0 + 63, j);
+        const Cell p0 = ld<EDGE>(d, s, c0 + 64, j);
+        L.hr[0][lane] = p0.h; L.hr[1][lane] = p0.hu; L.hr[2][lane] = p0.hv; L.hr[3][lane] = p0.b;
+        edge_tiny = tiny(a) || tiny(bl) || tiny(cc) || tiny(m1) || tiny(p0);
+        if (c0 + WAVE - 1 < d.nx) {
+            const Cell m2 = ld<EDGE>(d, s, c0 + 62, j);
+            const Cell p1 = ld<EDGE>(d, s, c0 + 65, j);
+            FState lo1, hi1, lo2, hi2;
+            cell_states(m2, m1, p0, lo1, hi1);
+            cell_states(m1, p0, p1, lo2, hi2);
+            const Face f = face_flux<HLLC, false, true>(g, hi1, m1.b, lo2, p0.b);
+            L.pf[0][lane] = f.fh; L.pf[1][lane] = f.fhu; L.pf[2][lane] = f.fhv;
+            L.pf[3][lane] = f.a; L.pf[4][lane] = f.c; L.pf[5][lane] = f.d;
+        }
     }
+    // rows of the segment whose edge cells hold tiny values
+    const unsigned long long edge_rows = __ballot(edge_tiny);
 
-    // (2) register window and the face below the first row.
-    Cell w0 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j0 - 2, s.eta_tidal);
-    Cell w1 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j0 - 1, s.eta_tidal);
-    Cell w2 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j0, s.eta_tidal);
-    Cell w3 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j0 + 1, s.eta_tidal);
-    Face fyb = face<HLLC, true>(g, w0, w1, w2, w3);
-    w0 = w1; w1 = w2; w2 = w3;
-    w3 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j0 + 2, s.eta_tidal);
-
-    // halo columns of the LDS row: lane 0 -> c0-2, 1 -> c0-1, 2 -> c0+64
-    const int hcol = lane == 0 ? c0 - 2 : (lane == 1 ? c0 - 1 : c0 + WAVE);
-    const int hidx = lane == 0 ? 0 : (lane == 1 ? 1 : WAVE + 2);
-    Cell halo;
-    if (lane < 3) halo = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, hcol, j0, s.eta_tidal);
-    Cell un;  // U^n at (c, j) for the RK2 average
+    // (2) y pipeline: top state of row j0-1, face below row j0
+    Cell rc, rn, rnn;
+    FState tc;
+    Face fyb;
+    bool t_prev, t_rc, t_rn, t_rnn;
+    {
+        const Cell a = ld<EDGE>(d, s, c, j0 - 2);
+        const Cell bm = ld<EDGE>(d, s, c, j0 - 1);
+        rc = ld<EDGE>(d, s, c, j0);
+        rn = ld<EDGE>(d, s, c, j0 + 1);
+        rnn = ld<EDGE>(d, s, c, j0 + 2);
+        FState lo, tm;
+        cell_states(a, bm, rc, lo, tm);
+        FState bc;
+        cell_states(bm, rc, rn, bc, tc);
+        fyb = face_flux<HLLC, true, true>(g, tm, bm.b, bc, rc.b);
+        t_prev = tiny(bm);
+        t_rc = tiny(rc);
+        t_rn = tiny(rn);
+        t_rnn = tiny(rnn);
+    }
     const bool own = c < d.nx;
+    Cell un;  // U^n at (c, j) for the RK2 average
+    un.b = 0.0;
     if (STAGE == 2 && own) {
         const long long k = (long long)j0 * pitch + c;
         un.h = d.ah[k]; un.hu = d.ahu[k]; un.hv = d.ahv[k];
@@ -259,61 +315,72 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     __syncthreads();
 
     for (int j = j0; j < j1; ++j) {
-        // prefetch for row j+1
+        const int r = j - j0;
         const bool more = j + 1 < j1;
-        Cell nw3, nhalo, nun;
+        Cell nxt, nun;
         if (more) {
-            nw3 = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j + 3, s.eta_tidal);
-            if (lane < 3) nhalo = load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, hcol, j + 1, s.eta_tidal);
+            nxt = ld<EDGE>(d, s, c, j + 3);
             if (STAGE == 2 && own) {
                 const long long k = (long long)(j + 1) * pitch + c;
                 nun.h = d.ah[k]; nun.hu = d.ahu[k]; nun.hv = d.ahv[k];
             }
         }
+        FState tn;
+        Face fyt;
+        auto row = [&](auto exact) {
+            constexpr bool EX = decltype(exact)::value;
+            // y: slope of row j+1 and the face between rows j and j+1
+            FState bn;
+            cell_states(rc, rn, rnn, bn, tn);
+            fyt = face_flux<HLLC, true, EX>(g, tc, rc.b, bn, rn.b);
 
-        // row j into LDS: own column at lane+2, halo columns from lanes 0..2
-        L.rh[lane + 2] = w1.h; L.rhu[lane + 2] = w1.hu; L.rhv[lane + 2] = w1.hv; L.rb[lane + 2] = w1.b;
-        if (lane < 3) {
-            L.rh[hidx] = halo.h; L.rhu[hidx] = halo.hu; L.rhv[hidx] = halo.hv; L.rb[hidx] = halo.b;
-        }
-        __syncthreads();
-        Cell xm2, xm1, xp1;
-        xm2.h = L.rh[lane]; xm2.hu = L.rhu[lane]; xm2.hv = L.rhv[lane]; xm2.b = L.rb[lane];
-        xm1.h = L.rh[lane + 1]; xm1.hu = L.rhu[lane + 1]; xm1.hv = L.rhv[lane + 1]; xm1.b = L.rb[lane + 1];
-        xp1.h = L.rh[lane + 3]; xp1.hu = L.rhu[lane + 3]; xp1.hv = L.rhv[lane + 3]; xp1.b = L.rb[lane + 3];
-        const int r = j - j0;
-        Face bnd;
-        bnd.fh = L.bnd[0][r]; bnd.fhu = L.bnd[1][r]; bnd.fhv = L.bnd[2][r];
-        bnd.a = L.bnd[3][r]; bnd.c = L.bnd[4][r]; bnd.d = L.bnd[5][r];
-        __syncthreads();
+            // x: slope of row j, face left of each cell, face right from the next lane
+            Cell xm = shfl_up1(rc);
+            Cell xp = shfl_down1(rc);
+            if (lane == 0) xm = Cell{L.hl[0][r], L.hl[1][r], L.hl[2][r], L.hl[3][r]};
+            if (lane == WAVE - 1) xp = Cell{L.hr[0][r], L.hr[1][r], L.hr[2][r], L.hr[3][r]};
+            FState xlo, xhi;
+            cell_states(xm, rc, xp, xlo, xhi);
+            FState rprev = shfl_up1(xhi);
+            if (lane == 0) rprev = FState{L.ph[0][r], L.ph[1][r], L.ph[2][r]};
+            const Face fxl = face_flux<HLLC, false, EX>(g, rprev, xm.b, xlo, rc.b);
+            Face fxr = shfl_down1(fxl);
+            if (lane == WAVE - 1)
+                fxr = Face{L.pf[0][r], L.pf[1][r], L.pf[2][r], L.pf[3][r], L.pf[4][r], L.pf[5][r]};
 
-        const Face fxl = face<HLLC, false>(g, xm2, xm1, w1, xp1);
-        const Face fyt = face<HLLC, true>(g, w0, w1, w2, w3);
-        Face fxr = shfl_down1(fxl);
-        if (lane == WAVE - 1) fxr = bnd;
-
-        if (own) {
-            double fh, fhu, fhv;
-            divergence(g, d.dx, d.dy, fxl, fxr, fyb, fyt, fh, fhu, fhv);
-            double h = w1.h, hu = w1.hu, hv = w1.hv;
-            stage_body(s.src, s.dt, fh, fhu, fhv, h, hu, hv);
-            if (STAGE == 2) {
-                // rki_kernel_fused.F90:435-463
-                h = 0.5 * (un.h + h);
-                hu = 0.5 * (un.hu + hu);
-                hv = 0.5 * (un.hv + hv);
-                clamp_desing(g, h, hu, hv);
-                smax = dmax(smax, cfl_speed(g, d.dx, d.dy, h, hu, hv));
+            if (own) {
+                double fh, fhu, fhv;
+                divergence<EX>(g, s.rdx, s.rdy, fxl, fxr, fyb, fyt, fh, fhu, fhv);
+                double h = rc.h, hu = rc.hu, hv = rc.hv;
+                stage_body<EX>(s.src, s.dt, fh, fhu, fhv, h, hu, hv);
+                if (STAGE == 2) {
+                    // rki_kernel_fused.F90:435-463
+                    h = 0.5 * (un.h + h);
+                    hu = 0.5 * (un.hu + hu);
+                    hv = 0.5 * (un.hv + hv);
+                    clamp_desing<EX>(g, h, hu, hv);
+                    smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
+                }
+                const long long k = (long long)j * pitch + c;
+                s.oh[k] = h; s.ohu[k] = hu; s.ohv[k] = hv;
             }
-            const long long k = (long long)j * pitch + c;
-            s.oh[k] = h; s.ohu[k] = hu; s.ohv[k] = hv;
-        }
+        };
+        const bool lane_tiny = t_prev || t_rc || t_rn || t_rnn || (STAGE == 2 && own && tiny(un));
+        if (__any(lane_tiny) || ((edge_rows >> r) & 1ull))
+            row(ExactTag<true>{});
+        else
+            row(ExactTag<false>{});
 
         fyb = fyt;
-        w0 = w1; w1 = w2; w2 = w3;
+        tc = tn;
+        rc = rn;
+        rn = rnn;
+        t_prev = t_rc;
+        t_rc = t_rn;
+        t_rn = t_rnn;
         if (more) {
-            w3 = nw3;
-            if (lane < 3) halo = nhalo;
+            rnn = nxt;
+            t_rnn = tiny(nxt);
             if (STAGE == 2) un = nun;
         }
     }

@@ -123,6 +123,27 @@ def test_coriolis_cfl_interval_dt_fixed():
                   tidal_constituents=[(0.3, 0.1, 2e-3), (0.2, 1.0, 5e-3)])
 
 
+def test_tiny_values_take_exact_division_rows():
+    """Cells holding non-zero values below 2^-300 (subnormal momenta, tiny
+    depths and beds) make their rows use plain IEEE division; results stay
+    bit-identical to the oracle."""
+    from rakali_amd import scenarios as S
+    scn = S.wetdry_islands(nx=130, ny=70, steps=30)
+    rng = np.random.default_rng(21)
+    hu = scn.hu.copy()
+    hv = scn.hv.copy()
+    idx = rng.choice(scn.nx * scn.ny, 200, replace=False)
+    vals = np.array([1e-305, -3e-310, 5e-320, -1e-301, 2.5e-308])
+    hu.reshape(-1, order="F")[idx[:100]] = rng.choice(vals, 100)
+    hv.reshape(-1, order="F")[idx[100:]] = rng.choice(vals, 100)
+    h = scn.h.copy()
+    h.reshape(-1, order="F")[idx[:20]] = 1e-303
+    b = scn.b.copy()
+    b.reshape(-1, order="F")[idx[20:30]] = -4e-310
+    scn.hu, scn.hv, scn.h, scn.b = (np.asfortranarray(a) for a in (hu, hv, h, b))
+    assert_parity(scn, 30, "tiny values")
+
+
 def test_t_end_bounds_dt():
     """rki_step passes dt_max = t_end - t (rki_api.F90:377): the last step
     lands on t_end exactly as in the oracle."""
