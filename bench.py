@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU work of the baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="size of the plain-copy bandwidth reference (0 disables)")
     return ap.parse_args()
 
 
@@ -92,6 +94,7 @@ def main():
     t_sim, dt, nsteps = g.clock
     mass = g.total_mass()
     g.close()
+    copy_gbs = copy_reference(args.copy_mib)
 
     out = {
         "metric": METRIC,
@@ -121,18 +124,44 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom),
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg[dom],
             "stage_avg_ms": {"1": avg[1], "2": avg[2]},
             "step_algorithmic_GBs": step_gbs,
             "step_frac": step_gbs / HBM_PEAK_GBS,
+            "copy_kernel_GBs": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs if copy_gbs else None,
         },
     }
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = run_cpu_baseline(scn, args.cpu_seconds, args.cpu_threads)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measured_traffic(stage: int):
+    """HBM bytes per launch of the stage kernel from the committed rocprofv3
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the same counters on a copy
+    kernel of known size); None when no summary is present."""
+    import re
+    p = ROOT / "profiles" / "stage_kernels_tidal4096.md"
+    if not p.exists():
+        return None
+    m = re.search(r"k_stage<%d>=(\d+)" % stage, p.read_text())
+    return int(m.group(1)) if m else None
+
+
+def copy_reference(mib: int):
+    """Achievable HBM bandwidth of a plain copy kernel (read + write bytes)."""
+    if mib <= 0:
+        return None
+    import ctypes
+    from rakali_amd import _lib
+    L = _lib.load()
+    gbs, ms = ctypes.c_double(), ctypes.c_double()
+    _lib.check("rkg_copy_bandwidth", L.rkg_copy_bandwidth(mib << 20, 20, ctypes.byref(gbs), ctypes.byref(ms)))
+    return gbs.value
 
 
 def run_cpu_baseline(scn, seconds: float, threads: int):

@@ -52,11 +52,20 @@ extern "C" {
 #define RKG_BC_CLAMPED 7
 #define RKG_BC_SPONGE 8
 
-/* Field selectors. */
+/* Field selectors.  H0/HU0/HV0 are U^n of the last step (the reference's
+ * RK save h0/hu0/hv0, rki_kernel_fused.F90:224-226), kept only while
+ * rkg_set_rk_save is on; HS/HUS/HVS are U*, the stage-1 result of the last
+ * step.  Fields above B are read-only. */
 #define RKG_FIELD_H 0
 #define RKG_FIELD_HU 1
 #define RKG_FIELD_HV 2
 #define RKG_FIELD_B 3
+#define RKG_FIELD_H0 4
+#define RKG_FIELD_HU0 5
+#define RKG_FIELD_HV0 6
+#define RKG_FIELD_HS 7
+#define RKG_FIELD_HUS 8
+#define RKG_FIELD_HVS 9
 
 #define RKG_MAX_TIDAL_CONSTITUENTS 10
 
@@ -128,6 +137,14 @@ int rkg_sync(rkg_grid *g);
 /* Rows per tile segment of the stage kernels (1..64; default 32). */
 int rkg_set_tile_rows(rkg_grid *g, int rows);
 int rkg_get_tile_rows(rkg_grid *g, int *rows);
+/* Keeps U^n of every step in three extra planes (24 B per cell per step of
+ * extra traffic) so that RKG_FIELD_H0..HV0 can be read; off by default. */
+int rkg_set_rk_save(rkg_grid *g, int enable);
+/* Reads / replaces the scalar configuration (nx and ny cannot change):
+ * boundary values, tide, friction, CFL, t_end, HLL/HLLC. */
+int rkg_get_config(rkg_grid *g, rkg_config *cfg);
+int rkg_set_config(rkg_grid *g, const rkg_config *cfg);
+
 /* Kernel timing: when enabled, rkg_step brackets each stage launch with
  * HIP events on the grid's stream and waits for every step, accumulating
  * the stage-1 and stage-2 kernel times (ms).  Enabling resets the sums. */
@@ -137,6 +154,12 @@ int rkg_profile_read(rkg_grid *g, double *stage1_ms, double *stage2_ms, long lon
 int rkg_get_stream(rkg_grid *g, void **stream);
 /* Last-error text of this thread. */
 const char *rkg_last_error(void);
+
+/* HBM bandwidth reference: times `reps` launches of a plain copy kernel
+ * (k_copy) of `bytes` bytes on the current device; *gbs counts read plus
+ * written bytes.  Used to calibrate traffic counters and state the
+ * achievable copy bandwidth next to the 8 TB/s nominal peak. */
+int rkg_copy_bandwidth(long long bytes, int reps, double *gbs, double *ms_per_copy);
 
 /* Evaluates the step's transcendental functions on n inputs, on the host
  * (on_device = 0) or the current GPU: fn 0 = x**(1.0/3.0) (Manning,

@@ -23,7 +23,8 @@ ERRORS = {-1: "not ready", -2: "argument out of range", -4: "HIP runtime error",
 
 BC = {"wall": 1, "open": 2, "tidal": 3, "inflow": 5, "discharge": 6,
       "clamped": 7, "sponge": 8}
-FIELDS = {"h": 0, "hu": 1, "hv": 2, "b": 3}
+FIELDS = {"h": 0, "hu": 1, "hv": 2, "b": 3, "h0": 4, "hu0": 5, "hv0": 6,
+          "h_star": 7, "hu_star": 8, "hv_star": 9}
 
 
 class Config(ctypes.Structure):
@@ -83,6 +84,10 @@ SIGNATURES = {
     "rkg_profile": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_profile_read": (ctypes.c_int, [_P, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "rkg_get_stream": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "rkg_copy_bandwidth": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp, _dp]),
+    "rkg_set_rk_save": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_get_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
+    "rkg_set_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
 }
 
 

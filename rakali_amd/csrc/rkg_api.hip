@@ -207,6 +207,14 @@ __global__ __launch_bounds__(256) void k_mass_final(const double* partial, int n
 
 constexpr int MASS_BLOCKS = 512;
 
+// Plain copy, 16 bytes per lane per iteration: the bandwidth reference
+// for the stage kernels and the calibration of the HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                              long long n2) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n2; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 __global__ void k_math_eval(int fn, const double* x, double* y, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -224,6 +232,7 @@ struct rkg_grid {
     long long pitch;
     size_t plane;        // elements per field plane
     double* buf;         // 7 planes: A(h,hu,hv) B(h,hu,hv) bed
+    double* save;        // 3 planes: U^n of the last step (RK save), or null
     CtrlSlot* ctrl;      // device [2]
     CtrlSlot* ctrl_host; // pinned [2]
     double* mass_buf;    // device [MASS_BLOCKS + 1]
@@ -249,6 +258,12 @@ double* field_ptr(rkg_grid* g, int field) {
     case RKG_FIELD_HU: return plane_ptr(g, 1);
     case RKG_FIELD_HV: return plane_ptr(g, 2);
     case RKG_FIELD_B: return plane_ptr(g, 6);
+    case RKG_FIELD_H0: return g->save ? g->save : nullptr;
+    case RKG_FIELD_HU0: return g->save ? g->save + g->plane : nullptr;
+    case RKG_FIELD_HV0: return g->save ? g->save + 2 * g->plane : nullptr;
+    case RKG_FIELD_HS: return plane_ptr(g, 3);
+    case RKG_FIELD_HUS: return plane_ptr(g, 4);
+    case RKG_FIELD_HVS: return plane_ptr(g, 5);
     default: return nullptr;
     }
 }
@@ -262,6 +277,9 @@ GridDesc make_desc(rkg_grid* g) {
     d.ah = plane_ptr(g, 0); d.ahu = plane_ptr(g, 1); d.ahv = plane_ptr(g, 2);
     d.bh = plane_ptr(g, 3); d.bhu = plane_ptr(g, 4); d.bhv = plane_ptr(g, 5);
     d.bed = plane_ptr(g, 6);
+    d.sh = g->save;
+    d.shu = g->save ? g->save + g->plane : nullptr;
+    d.shv = g->save ? g->save + 2 * g->plane : nullptr;
     d.g = c.g; d.cfl = c.cfl; d.manning_n = c.manning_n; d.coriolis_f = c.coriolis_f;
     d.use_hllc = c.use_hllc;
     for (int s = 0; s < 4; ++s) d.bc[s] = c.bc[s];
@@ -431,6 +449,7 @@ void rkg_destroy(rkg_grid** pg) {
     if (g->magic != GRID_MAGIC) { *pg = nullptr; return; }
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->buf) (void)hipFree(g->buf);
+    if (g->save) (void)hipFree(g->save);
     if (g->ctrl) (void)hipFree(g->ctrl);
     if (g->mass_buf) (void)hipFree(g->mass_buf);
     if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
@@ -446,7 +465,8 @@ void rkg_destroy(rkg_grid** pg) {
 static int copy_field(rkg_grid* g, int field, double* host_or_dev, int ld, bool upload, hipMemcpyKind kind) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     double* f = field_ptr(g, field);
-    if (!f) { set_error("unknown field %d", field); return RKG_ERR_RANGE; }
+    if (!f) { set_error("unknown or unallocated field %d", field); return RKG_ERR_RANGE; }
+    if (upload && field > RKG_FIELD_B) { set_error("field %d is read-only", field); return RKG_ERR_RANGE; }
     if (!host_or_dev || ld < g->cfg.nx) { set_error("bad buffer or leading dimension"); return RKG_ERR_RANGE; }
     const size_t w = (size_t)g->cfg.nx * sizeof(double);
     if (upload) {
@@ -563,6 +583,40 @@ int rkg_set_tile_rows(rkg_grid* g, int rows) {
     return RKG_OK;
 }
 
+int rkg_set_rk_save(rkg_grid* g, int enable) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (enable && !g->save) {
+        HIP_TRY(hipMalloc((void**)&g->save, 3 * g->plane * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(g->save, 0, 3 * g->plane * sizeof(double), g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    } else if (!enable && g->save) {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipFree(g->save));
+        g->save = nullptr;
+    }
+    return RKG_OK;
+}
+
+int rkg_get_config(rkg_grid* g, rkg_config* cfg) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!cfg) return RKG_ERR_RANGE;
+    *cfg = g->cfg;
+    return RKG_OK;
+}
+
+int rkg_set_config(rkg_grid* g, const rkg_config* cfg) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!cfg) return RKG_ERR_RANGE;
+    if (cfg->nx != g->cfg.nx || cfg->ny != g->cfg.ny) { set_error("nx and ny cannot change"); return RKG_ERR_RANGE; }
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    const bool speed = cfg->g != g->cfg.g || cfg->dx != g->cfg.dx || cfg->dy != g->cfg.dy;
+    g->cfg = *cfg;
+    if (speed) g->speed_stale = true;
+    return RKG_OK;
+}
+
 int rkg_profile(rkg_grid* g, int enable) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     for (int k = 0; k < 3; ++k)
@@ -596,6 +650,40 @@ int rkg_get_tile_rows(rkg_grid* g, int* rows) {
 }
 
 const char* rkg_last_error(void) { return g_last_error.c_str(); }
+
+int rkg_copy_bandwidth(long long bytes, int reps, double* gbs, double* ms_per_copy) {
+    if (bytes < 4096 || reps < 1 || !gbs) { set_error("bad arguments"); return RKG_ERR_RANGE; }
+    const long long n2 = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipMalloc((void**)&a, n2 * sizeof(double2)));
+    hipError_t e = hipMalloc((void**)&b, n2 * sizeof(double2));
+    if (e == hipSuccess) e = hipMemset(a, 0, n2 * sizeof(double2));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float ms = 0.f;
+    if (e == hipSuccess) {
+        const int blocks = 256 * 8 * 4;  // 8 waves per CU per XCD slot, grid-stride
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n2);  // warm-up
+        e = hipEventRecord(e0, 0);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) {
+            hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n2);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) { set_error("rkg_copy_bandwidth: %s", hipGetErrorString(e)); return RKG_ERR_HIP; }
+    const double per = (double)ms / reps;
+    *gbs = 2.0 * (double)(n2 * 16) / (per * 1e-3) / 1e9;   // read + write
+    if (ms_per_copy) *ms_per_copy = per;
+    return RKG_OK;
+}
 
 int rkg_math_eval(int fn, const double* x, double* y, int n, int on_device) {
     if (fn < 0 || fn > 2 || n < 0 || (n > 0 && (!x || !y))) { set_error("bad arguments"); return RKG_ERR_RANGE; }

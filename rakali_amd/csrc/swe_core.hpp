@@ -48,9 +48,9 @@ struct Face {
     double fh, fhu, fhv, a, c, d;
 };
 
-__device__ __forceinline__ double fmax0(double x) { return x > 0.0 ? x : 0.0; }
-__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ double fmax0(double x) { return x > 0.0 ? x : 0.0; }
+__host__ __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 // x**4 as LLVM flang lowers it for a run-time base: ((x*x)*x)*x.
 __device__ __forceinline__ double pow4(double x) { return ((x * x) * x) * x; }
 

@@ -35,6 +35,7 @@ struct GridDesc {
     double dx, dy;
     double* ah; double* ahu; double* ahv;   // A: U^n / U^{n+1}
     double* bh; double* bhu; double* bhv;   // B: U*
+    double* sh; double* shu; double* shv;   // optional copy of U^n (RK save), or null
     const double* bed;
     double g, cfl, manning_n, coriolis_f;
     int use_hllc;
@@ -56,7 +57,7 @@ struct EdgeEta {
 };
 
 // Tidal surface, rki_boundary.F90:92-101.
-__device__ __forceinline__ double tidal_eta(const GridDesc& d, double time) {
+__host__ __device__ __forceinline__ double tidal_eta(const GridDesc& d, double time) {
     double e = 0.0;
     if (d.n_tc > 0) {
         for (int n = 0; n < d.n_tc; ++n) e = e + d.tc_amp[n] * crm::cos(d.tc_omega[n] * time - d.tc_phase[n]);
@@ -79,7 +80,7 @@ __device__ __forceinline__ Cell load_direct(const double* __restrict__ h, const 
 // `nrm_sign` is -1 on W/S and +1 on E/N; `normal_is_u` on W/E.
 // kk = ghost layer (0 = adjacent), mir = mirror cell for wall edges,
 // in0 = first interior cell.  rki_boundary.F90:220-466.
-__device__ __forceinline__ Cell ghost_from(const GridDesc& d, int tag, double eta_tidal,
+__host__ __device__ __forceinline__ Cell ghost_from(const GridDesc& d, int tag, double eta_tidal,
                                            const Cell& mir, const Cell& in0, double nrm_sign,
                                            bool normal_is_u) {
     Cell g;
@@ -363,6 +364,9 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                 }
                 const long long k = (long long)j * pitch + c;
                 s.oh[k] = h; s.ohu[k] = hu; s.ohv[k] = hv;
+                if (STAGE == 1 && d.sh != nullptr) {  // h0, hu0, hv0 (rki_kernel_fused.F90:224-226)
+                    d.sh[k] = rc.h; d.shu[k] = rc.hu; d.shv[k] = rc.hv;
+                }
             }
         };
         const bool lane_tiny = t_prev || t_rc || t_rn || t_rnn || (STAGE == 2 && own && tiny(un));
