@@ -1,4 +1,6 @@
-"""Build recipe for the in-tree HIP library ``rakali_amd/librakali_amd.so``.
+"""Build recipe for the in-tree libraries ``rakali_amd/librakali_amd.so``
+(the HIP step, ``rkg_*``) and ``rakali_amd/librakali.so`` (the handle API,
+``rki_*``, host code linked against the former).
 
 The library is compiled for gfx950 only.  Device code is built with
 ``-ffp-contract=off`` so that no multiply and add are fused: every FP64
@@ -21,6 +23,11 @@ LIB = PKG / "librakali_amd.so"
 SOURCES = [CSRC / "rkg_api.hip"]
 HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp",
            ROOT / "include" / "rakali_amd.h"]
+RKI_LIB = PKG / "librakali.so"
+RKI_SOURCES = [CSRC / "rki_api.cpp"]
+RKI_HEADERS = [CSRC / "crmath.hpp", ROOT / "include" / "rakali.h", ROOT / "include" / "rakali_amd.h"]
+DRIVER = PKG / "rakali_driver"
+DRIVER_SOURCES = [PKG / "fortran" / "rakali_amd_api.F90", PKG / "fortran" / "rakali_driver.F90"]
 ARCH = "gfx950"
 
 
@@ -29,6 +36,34 @@ def hipcc() -> str:
         if cand and Path(cand).exists():
             return cand
     raise RuntimeError("hipcc not found; ROCm is required to build rakali_amd")
+
+
+def flang() -> str | None:
+    for cand in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang"),
+                 shutil.which("flang")):
+        if cand and Path(cand).exists():
+            return cand
+    return None
+
+
+def build_driver(verbose: bool = False) -> Path | None:
+    """The Fortran namelist driver (iso_c_binding over librakali.so), or
+    None when no Fortran compiler is installed."""
+    fc = flang()
+    if fc is None:
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod_dir:
+        tmp = DRIVER.with_suffix(".tmp")
+        cmd = [fc, "-O2", "-module-dir", mod_dir, "-o", str(tmp), *map(str, DRIVER_SOURCES),
+               "-L", str(PKG), "-l:librakali.so", "-l:librakali_amd.so", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"{fc} failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+        os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 def compile_flags() -> list[str]:
@@ -40,24 +75,38 @@ def compile_flags() -> list[str]:
     ]
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile the library unconditionally and return its path."""
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *compile_flags(), "-o", str(tmp), *map(str, SOURCES)]
+def _run(cmd: list[str], out: Path, verbose: bool) -> Path:
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [*cmd, "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+    return out
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile both libraries unconditionally; return the step library's path."""
+    _run([hipcc(), *compile_flags(), *map(str, SOURCES)], LIB, verbose)
+    # The handle API is host code (it uses the step's host-side sin/cos);
+    # it links the step library by soname and finds it next to itself.
+    _run([hipcc(), *compile_flags(), "-x", "hip", *map(str, RKI_SOURCES),
+          "-L", str(PKG), "-l:librakali_amd.so", "-Wl,-rpath,$ORIGIN",
+          "-I", str(CSRC)], RKI_LIB, verbose)
+    build_driver(verbose)
     return LIB
 
 
 def is_stale() -> bool:
-    if not LIB.exists():
-        return True
-    t = LIB.stat().st_mtime
-    return any(p.stat().st_mtime > t for p in SOURCES + HEADERS)
+    for lib, srcs in ((LIB, SOURCES + HEADERS), (RKI_LIB, RKI_SOURCES + RKI_HEADERS + [LIB])):
+        if not lib.exists():
+            return True
+        t = lib.stat().st_mtime
+        if any(p.exists() and p.stat().st_mtime > t for p in srcs):
+            return True
+    return False
 
 
 if __name__ == "__main__":
