@@ -13,6 +13,9 @@ stage kernel (kernel time from HIP events on the launching stream) and a
 CPU baseline (the oracle, OpenMP, on a bounded sample of the same run).
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+
+An "ensemble" object adds the batched-vs-loop comparison on 64 uneven
+tidal basins (500 steps each).
 """
 from __future__ import annotations
 
@@ -49,10 +52,15 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain-copy bandwidth reference (0 disables)")
+    ap.add_argument("--ensemble-members", type=int, default=64,
+                    help="members of the uneven-ensemble comparison (0 disables)")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     return ap.parse_args()
 
 
 def main():
+    import faulthandler
+    faulthandler.enable()
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build's path is "
@@ -134,6 +142,8 @@ def main():
             "frac_of_copy": achieved / copy_gbs if copy_gbs else None,
         },
     }
+    if args.ensemble_members > 0:
+        out["ensemble"] = run_ensemble(args.ensemble_members, args.ensemble_steps)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = run_cpu_baseline(scn, args.cpu_seconds, args.cpu_threads)
     print(json.dumps(out), flush=True)
@@ -150,6 +160,82 @@ def measured_traffic(stage: int):
         return None
     m = re.search(r"k_stage<%d>=(\d+)" % stage, p.read_text())
     return int(m.group(1)) if m else None
+
+
+def run_ensemble(n_members: int, steps: int, warmup: int = 2):
+    """Batched vs looped: `n_members` uneven tidal basins (nx, ny log-uniform
+    in [128, 2048], dx = 25 m, own bed, tide and Manning n), `steps` steps of
+    every member.  Batched = one launch per RK stage for the whole ensemble
+    (schedule from the autotuner, steps replayed from a hipGraph).  Loop =
+    one grid per member, all on one stream, each member's steps launched
+    back to back.  Both start from the same state and take the same steps;
+    the final depths must agree bit for bit."""
+    import ctypes
+    import zlib
+
+    import numpy as np
+
+    from rakali_amd import Grid, _lib
+    from rakali_amd import scenarios as S
+    from rakali_amd.ensemble import from_scenarios
+
+    scns = S.tidal_ensemble(n=n_members, seed=4, size=(128, 2048), dx=25.0, steps=steps)
+    cells = sum(s.nx * s.ny for s in scns)
+
+    e = from_scenarios(scns)
+    tune_ms = e.autotune(steps=3)
+    sched = e.schedule
+    e.set_schedule(sched["schedule"], use_graph=True)
+    e.step(warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    batched_s = time.perf_counter() - t0
+    crc_batched = 0
+    for m in range(len(scns)):
+        crc_batched = zlib.crc32(e.get_field(m, "h").tobytes(), crc_batched)
+    e.close()
+
+    L = _lib.load()
+    grids = []
+    stream = None
+    for s in scns:
+        g = Grid(s.nx, s.ny, s.dx, s.dy, stream=stream, **s.grid_kw)
+        for k in ("b", "h", "hu", "hv"):
+            g.set_field(k, getattr(s, k))
+        if stream is None:
+            p = ctypes.c_void_p()
+            _lib.check("rkg_get_stream", L.rkg_get_stream(g.handle, ctypes.byref(p)))
+            stream = p.value
+        grids.append(g)
+    for g in grids:
+        g.step(warmup)
+    grids[0].sync()
+    t0 = time.perf_counter()
+    for g in grids:
+        g.step(steps)
+    grids[0].sync()
+    loop_s = time.perf_counter() - t0
+    crc_loop = 0
+    for g in grids:
+        crc_loop = zlib.crc32(g.get_field("h").tobytes(), crc_loop)
+    # grids[0] owns the shared stream: close it last
+    for g in reversed(grids):
+        g.close()
+
+    return {
+        "workload": f"{n_members} tidal basins, nx, ny log-uniform in [128, 2048], dx=25 m, own bed, "
+                    f"tide A~U[0.5,2] m, Manning n~U[0.015,0.04], HLLC, {steps} steps each",
+        "members": n_members, "cells": cells, "steps": steps,
+        "batched": {"value": cells * steps / batched_s, "unit": "cell-updates/s",
+                    "ms_per_step": batched_s / steps * 1e3, "schedule": sched,
+                    "autotune_ms_per_step": tune_ms, "graph": True},
+        "loop": {"value": cells * steps / loop_s, "unit": "cell-updates/s",
+                 "ms_per_step": loop_s / steps * 1e3, "launches_per_step": 2 * n_members},
+        "speedup": loop_s / batched_s,
+        "identical_results": crc_batched == crc_loop,
+    }
 
 
 def copy_reference(mib: int):

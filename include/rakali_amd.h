@@ -100,7 +100,9 @@ typedef struct rkg_grid rkg_grid;   /* opaque handle */
 int rkg_config_default(rkg_config *cfg);
 
 /* Allocates device state for one grid on the current HIP device.  `stream`
- * is a hipStream_t to launch on, or NULL for a library-owned stream. */
+ * is a hipStream_t to launch on, or NULL for a library-owned stream.  A
+ * caller-provided stream (including one obtained from another grid with
+ * rkg_get_stream) must outlive every grid that uses it. */
 int rkg_create(const rkg_config *cfg, void *stream, rkg_grid **out);
 /* Frees the grid and nulls *g; a NULL or already-destroyed handle is a no-op. */
 void rkg_destroy(rkg_grid **g);
@@ -167,6 +169,37 @@ int rkg_copy_bandwidth(long long bytes, int reps, double *gbs, double *ms_per_co
  * rki_boundary.F90:96-100, Coriolis rki_kernel_fused.F90:212-213).  Both
  * sides return identical, correctly rounded bits. */
 int rkg_math_eval(int fn, const double *x, double *y, int n, int on_device);
+
+/* ------------------------------------------------------------------------
+ * Ensembles of independent grids of uneven sizes, stepped together with
+ * one launch per RK stage.  No counterpart in the reference, which steps
+ * one grid per handle; these are additive entry points for batched runs.
+ * Each member keeps its own clock (t, dt, step count), bed, physics and
+ * boundaries; results per member are bit-identical to a single grid with
+ * the same configuration.  All members share use_hllc.
+ * ------------------------------------------------------------------------ */
+typedef struct rkg_ensemble rkg_ensemble;   /* opaque handle */
+
+int rkg_ensemble_create(const rkg_config *cfgs, int n_members, void *stream, rkg_ensemble **out);
+void rkg_ensemble_destroy(rkg_ensemble **e);
+/* member count, total interior cells and tiles in the table */
+int rkg_ensemble_size(rkg_ensemble *e, int *n_members, long long *cells, int *n_tiles);
+/* fields as for rkg_upload / rkg_download (H, HU, HV, B writable; HS.. readable) */
+int rkg_ensemble_upload(rkg_ensemble *e, int member, int field, const double *src, int ld);
+int rkg_ensemble_download(rkg_ensemble *e, int member, int field, double *dst, int ld);
+/* n steps of every member (each bounded by its own t_end, as rkg_step) */
+int rkg_ensemble_step(rkg_ensemble *e, int n);
+int rkg_ensemble_sync(rkg_ensemble *e);
+int rkg_ensemble_get_clock(rkg_ensemble *e, int member, double *t, double *dt, long long *step_count);
+int rkg_ensemble_total_mass(rkg_ensemble *e, int member, double *mass);
+/* schedule: 0 = one workgroup per tile, 1 = persistent workgroups taking
+ * tiles from an atomic queue; tile_rows in [4, 64] (0 = keep);
+ * persistent_blocks (0 = keep); use_graph: replay steps from a hipGraph. */
+int rkg_ensemble_set_schedule(rkg_ensemble *e, int schedule, int tile_rows, int persistent_blocks, int use_graph);
+int rkg_ensemble_get_schedule(rkg_ensemble *e, int *schedule, int *tile_rows, int *persistent_blocks, int *use_graph);
+/* Times `steps` steps of each candidate schedule and tile height on the
+ * live ensemble, keeps the fastest, restores the state. */
+int rkg_ensemble_autotune(rkg_ensemble *e, int steps, double *best_ms_per_step);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,19 @@ SIGNATURES = {
     "rkg_set_rk_save": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_get_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_set_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
+    "rkg_ensemble_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, _P, ctypes.POINTER(_P)]),
+    "rkg_ensemble_destroy": (None, [ctypes.POINTER(_P)]),
+    "rkg_ensemble_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong),
+                                         ctypes.POINTER(ctypes.c_int)]),
+    "rkg_ensemble_upload": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int]),
+    "rkg_ensemble_download": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int]),
+    "rkg_ensemble_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_ensemble_sync": (ctypes.c_int, [_P]),
+    "rkg_ensemble_get_clock": (ctypes.c_int, [_P, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "rkg_ensemble_total_mass": (ctypes.c_int, [_P, ctypes.c_int, _dp]),
+    "rkg_ensemble_set_schedule": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "rkg_ensemble_get_schedule": (ctypes.c_int, [_P] + [ctypes.POINTER(ctypes.c_int)] * 4),
+    "rkg_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_int, _dp]),
 }
 
 

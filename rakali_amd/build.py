@@ -20,9 +20,9 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "librakali_amd.so"
-SOURCES = [CSRC / "rkg_api.hip"]
+SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip"]
 HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp",
-           ROOT / "include" / "rakali_amd.h"]
+           CSRC / "stage_kernels.hpp", CSRC / "rkg_internal.hpp", ROOT / "include" / "rakali_amd.h"]
 RKI_LIB = PKG / "librakali.so"
 RKI_SOURCES = [CSRC / "rki_api.cpp"]
 RKI_HEADERS = [CSRC / "crmath.hpp", ROOT / "include" / "rakali.h", ROOT / "include" / "rakali_amd.h"]

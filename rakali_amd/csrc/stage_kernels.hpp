@@ -1,0 +1,182 @@
+// stage_kernels.hpp -- the stage kernels and the small reductions shared by
+// the single-grid ABI (rkg_api.hip) and the ensemble ABI
+// (rkg_ensemble.hip).  Kernels have internal linkage: each translation unit
+// that launches them holds its own copy.
+#pragma once
+
+#include "swe_kernels.hpp"
+
+#ifndef RKG_WAVES_PER_EU
+#define RKG_WAVES_PER_EU 3
+#endif
+
+namespace rk {
+namespace {
+
+constexpr int MODE_STEP = 0;
+constexpr int MODE_ADVANCE = 1;
+constexpr int MASS_BLOCKS = 512;
+
+// dt of a step from the control slot of the state it starts from:
+// rki_solver.F90:728-745 with rki_api.F90:377 (step) / :402 (advance).
+__device__ __forceinline__ double step_dt(const GridDesc& d, const CtrlSlot& cp, double dt_max) {
+    if (d.dt_fixed > 0.0) return dmin(d.dt_fixed, dt_max);
+    if (cp.step_count % d.cfl_interval == 0 || cp.dt <= 0.0) {
+        const double max_speed = __longlong_as_double((long long)cp.speed_bits);
+        double dt;
+        if (max_speed > 0.0)
+            dt = d.cfl / max_speed;
+        else
+            dt = d.cfl * dmin(d.dx, d.dy) / sqrt(d.g * DRY_TOLERANCE);
+        dt = dt * 1.0;  // ssp_coeff of RK2
+        return dmin(dt, dt_max);
+    }
+    return cp.dt;
+}
+
+__device__ __forceinline__ SourceTerms make_src(const GridDesc& d, double dt) {
+    SourceTerms s;
+    s.g = d.g;
+    s.n_sq = d.manning_n * d.manning_n;
+    s.has_friction = d.manning_n > 0.0;
+    s.has_coriolis = fabs(d.coriolis_f) > 1.0e-20;
+    s.cosf = 0.0;
+    s.sinf = 0.0;
+    if (s.has_coriolis) {
+        s.cosf = crm::cos(d.coriolis_f * dt);
+        s.sinf = crm::sin(d.coriolis_f * dt);
+    }
+    return s;
+}
+
+__device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, int seg) {
+    const int c0 = strip * WAVE;
+    const int j0 = seg * d.ty;
+    const int j1 = min(j0 + d.ty, d.ny);
+    return c0 >= 2 && c0 + WAVE + 1 < d.nx && j0 >= 2 && j1 + 1 < d.ny;
+}
+
+// One tile of one stage of one grid.  `where(d)` gives the tile as
+// (strip, segment, leader); the leader is the single tile per grid and
+// launch that advances the control slot.
+template <int STAGE, bool HLLC, class Where>
+__device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p, int mode, double tlim,
+                                           Where where) {
+    const int3 w = where(d);
+    const bool leader = w.z != 0;
+    StageCtx s;
+    if (STAGE == 1) {
+        const CtrlSlot cp = d.ctrl[p];
+        const bool active = (mode == MODE_STEP) || (cp.t < tlim - 1.0e-10);
+        const double dt_max = tlim - cp.t;
+        const double dt = active ? step_dt(d, cp, dt_max) : cp.dt;
+        if (leader && threadIdx.x == 0) {
+            CtrlSlot* q = &d.ctrl[1 - p];
+            q->t = active ? cp.t + dt : cp.t;
+            q->dt = dt;
+            q->step_count = active ? cp.step_count + 1 : cp.step_count;
+            q->speed_bits = active ? 0ull : cp.speed_bits;
+            q->active = active ? 1 : 0;
+        }
+        if (!active) return;
+        s.ih = d.ah; s.ihu = d.ahu; s.ihv = d.ahv;
+        s.oh = d.bh; s.ohu = d.bhu; s.ohv = d.bhv;
+        s.dt = dt;
+        s.eta_tidal = tidal_eta(d, cp.t);
+    } else {
+        const CtrlSlot cq = d.ctrl[1 - p];
+        if (!cq.active) return;
+        s.ih = d.bh; s.ihu = d.bhu; s.ihv = d.bhv;
+        s.oh = d.ah; s.ohu = d.ahu; s.ohv = d.ahv;
+        s.dt = cq.dt;
+        s.eta_tidal = tidal_eta(d, cq.t);  // bc%time = t + dt
+    }
+    s.src = make_src(d, s.dt);
+    s.rdx = recip(d.dx);
+    s.rdy = recip(d.dy);
+    const int strip = w.x, seg = w.y;
+    double smax;
+    if (tile_is_interior(d, strip, seg))
+        smax = process_tile<STAGE, HLLC, false>(d, s, L, strip, seg);
+    else
+        smax = process_tile<STAGE, HLLC, true>(d, s, L, strip, seg);
+    if (STAGE == 2) {
+        smax = wave_max(smax);
+        if (threadIdx.x == 0 && smax > 0.0)
+            atomicMax(&d.ctrl[1 - p].speed_bits, (unsigned long long)__double_as_longlong(smax));
+    }
+}
+
+// Single grid: one workgroup (one wavefront) per tile.
+template <int STAGE, bool HLLC>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage(GridDesc d, int p, int mode, double tlim) {
+    __shared__ TileLds L;
+    stage_tile<STAGE, HLLC>(d, L, p, mode, tlim, [](const GridDesc& g) {
+        const int tile = xcd_tile(blockIdx.x, g.nstrips * g.nsegs);
+        return make_int3(tile % g.nstrips, tile / g.nstrips, blockIdx.x == 0);
+    });
+}
+
+// Resets control slot p to time t / step count n with an empty speed max.
+__global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long long n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctrl[p].t = t;
+        ctrl[p].dt = dt;
+        ctrl[p].step_count = n;
+        ctrl[p].speed_bits = 0ull;
+        ctrl[p].active = 1;
+    }
+}
+
+// Max CFL speed of the state A into slot p (compute_timestep's reduction,
+// rki_kernel_timestep.F90:53-70).
+__global__ __launch_bounds__(256) void k_init_speed(GridDesc d, int p) {
+    double m = 0.0;
+    const Recip rdx = recip(d.dx), rdy = recip(d.dy);
+    const long long n = (long long)d.nx * d.ny;
+    for (long long k = blockIdx.x * 256ll + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
+        const long long o = (long long)j * d.pitch + i;
+        m = dmax(m, cfl_speed<true>(d.g, rdx, rdy, d.ah[o], d.ahu[o], d.ahv[o]));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && m > 0.0)
+        atomicMax(&d.ctrl[p].speed_bits, (unsigned long long)__double_as_longlong(m));
+}
+
+// Mass partials: block b sums rows b, b + gridDim.x, ... in a fixed order
+// (compute_total_mass, rki_init.F90:287-317).
+__global__ __launch_bounds__(256) void k_mass_partial(GridDesc d, double* partial) {
+    __shared__ double red[256];
+    const double area = d.dx * d.dy;
+    double acc = 0.0;
+    for (int j = blockIdx.x; j < d.ny; j += gridDim.x) {
+        for (int i = threadIdx.x; i < d.nx; i += 256) {
+            const double h = d.ah[(long long)j * d.pitch + i];
+            if (h > 0.0 && h < 1.0e6) acc = acc + h * area;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_mass_final(const double* partial, int n, double* out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc = acc + partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+}  // namespace
+}  // namespace rk

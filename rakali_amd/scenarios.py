@@ -109,3 +109,39 @@ def wetdry_islands(nx: int = 160, ny: int = 96, dx: float = 10.0, seed: int = 3,
     kw = dict(cfl=0.45, manning_n=0.025, use_hllc=use_hllc, bc=tuple(bc), t_end=1.0e12,
               tidal_amplitude=0.5, tidal_period=600.0)
     return Scenario("wetdry_islands", nx, ny, dx, dx, b, h, z, z.copy(), steps, kw)
+
+
+def tidal_ensemble(n: int = 64, seed: int = 4, size: Tuple[int, int] = (128, 2048), dx: float = 25.0,
+                   steps: int = 500) -> list:
+    """Ensemble of uneven tidal basins: nx and ny log-uniform in `size`
+    (rounded to integers), dx = dy = 25 m; each member has its own bed from
+    the tidal-basin recipe (mean depth 30 m, 64 bumps A~U[-20,35] m,
+    bed seed = 1000 * seed + member), a west tide of amplitude U[0.5, 2.0] m
+    (T = 44712 s), walls elsewhere, Manning n U[0.015, 0.04], HLLC, CFL 0.45,
+    still water.  Draw order from PCG64(seed): nx (n values), ny (n),
+    amplitude (n), Manning n (n).
+
+    Bump widths are U[200, 4000] m times the member's linear size relative
+    to the 4096 x 20 m basin, sqrt(nx ny) dx / 81920 m: 64 bumps of the
+    basin's widths on a member a few km across would cover it entirely and
+    leave it dry land, while the scaled widths keep the basin's bump density
+    and its wet/dry mix."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = np.log(size[0]), np.log(size[1])
+    nxs = np.rint(np.exp(rng.uniform(lo, hi, n))).astype(int)
+    nys = np.rint(np.exp(rng.uniform(lo, hi, n))).astype(int)
+    amps = rng.uniform(0.5, 2.0, n)
+    mans = rng.uniform(0.015, 0.04, n)
+    out = []
+    for m in range(n):
+        scale = np.sqrt(float(nxs[m]) * float(nys[m])) * dx / (4096 * 20.0)
+        b = gaussian_bed(int(nxs[m]), int(nys[m]), dx, dx, -30.0, 64, (-20.0, 35.0),
+                         (200.0 * scale, 4000.0 * scale), 1000 * seed + m)
+        h = still_water(b, 0.0)
+        z = np.zeros_like(h, order="F")
+        kw = dict(cfl=0.45, manning_n=float(mans[m]), use_hllc=True,
+                  bc=("tidal", "wall", "wall", "wall"), tidal_amplitude=float(amps[m]),
+                  tidal_period=TIDAL_PERIOD, t_end=1.0e12)
+        out.append(Scenario(f"tidal_member_{m}", int(nxs[m]), int(nys[m]), dx, dx, b, h, z, z.copy(),
+                            steps, kw))
+    return out

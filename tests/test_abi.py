@@ -1,6 +1,8 @@
 """The C-ABI library loads and exports every symbol include/rakali_amd.h
 declares, and the host-side entry points behave without a GPU."""
 import ctypes
+
+import numpy as np
 import re
 from pathlib import Path
 
@@ -60,3 +62,36 @@ def test_python_config_validation():
         make_config(8, 8, 1.0, 1.0, bc=("nested", "wall", "wall", "wall"))
     c = make_config(8, 8, 1.0, 1.0, tidal_constituents=[(0.5, 0.1, 1e-4)])
     assert c.n_tidal_constituents == 1 and c.tidal_omega[0] == 1e-4
+
+
+def test_ensemble_arguments_without_a_gpu():
+    from rakali_amd import _lib
+    L = _lib.load()
+    out = ctypes.c_void_p()
+    assert L.rkg_ensemble_step(None, 1) == -100
+    assert L.rkg_ensemble_sync(None) == -100
+    assert L.rkg_ensemble_create(None, 0, None, ctypes.byref(out)) == -2
+    cfgs = (_lib.Config * 2)()
+    for c in cfgs:
+        L.rkg_config_default(ctypes.byref(c))
+        c.nx, c.ny = 16, 16
+    cfgs[1].use_hllc = 1            # members must share the Riemann solver
+    assert L.rkg_ensemble_create(cfgs, 2, None, ctypes.byref(out)) == -5
+    cfgs[1].use_hllc = 0
+    cfgs[1].ny = 1
+    assert L.rkg_ensemble_create(cfgs, 2, None, ctypes.byref(out)) == -2
+    assert out.value is None
+    null = ctypes.c_void_p()
+    L.rkg_ensemble_destroy(ctypes.byref(null))
+
+
+def test_tidal_ensemble_recipe():
+    from rakali_amd import scenarios as S
+    a = S.tidal_ensemble(n=6, seed=4, size=(128, 2048), steps=0)
+    b = S.tidal_ensemble(n=6, seed=4, size=(128, 2048), steps=0)
+    assert [(s.nx, s.ny) for s in a] == [(s.nx, s.ny) for s in b]
+    for s in a:
+        assert 128 <= s.nx <= 2048 and 128 <= s.ny <= 2048
+        assert 0.5 <= s.grid_kw["tidal_amplitude"] <= 2.0
+        assert 0.015 <= s.grid_kw["manning_n"] <= 0.04
+        assert s.b.shape == (s.nx, s.ny) and np.array_equal(s.h, np.maximum(-s.b, 0.0))
