@@ -15,7 +15,9 @@ CPU baseline (the oracle, OpenMP, on a bounded sample of the same run).
     python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
 
 An "ensemble" object adds the batched-vs-loop comparison on 64 uneven
-tidal basins (500 steps each).
+tidal basins (500 steps each); an "adjoint" object the cost of the gauge-
+misfit gradient (w.r.t. bed and per-cell Manning n) on a 1024 x 1024 basin
+over 500 steps against the forward run, and its central-difference check.
 """
 from __future__ import annotations
 
@@ -55,6 +57,9 @@ def parse():
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the uneven-ensemble comparison (0 disables)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--adjoint-n", type=int, default=1024, help="adjoint basin size (0 disables)")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--adjoint-fd", type=int, default=32, help="finite-difference directions")
     return ap.parse_args()
 
 
@@ -144,6 +149,8 @@ def main():
     }
     if args.ensemble_members > 0:
         out["ensemble"] = run_ensemble(args.ensemble_members, args.ensemble_steps)
+    if args.adjoint_n > 0:
+        out["adjoint"] = run_adjoint(args.adjoint_n, args.adjoint_steps, args.adjoint_fd)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = run_cpu_baseline(scn, args.cpu_seconds, args.cpu_threads)
     print(json.dumps(out), flush=True)
@@ -235,6 +242,93 @@ def run_ensemble(n_members: int, steps: int, warmup: int = 2):
                  "ms_per_step": loop_s / steps * 1e3, "launches_per_step": 2 * n_members},
         "speedup": loop_s / batched_s,
         "identical_results": crc_batched == crc_loop,
+    }
+
+
+def run_adjoint(n: int, steps: int, fd_dirs: int, checkpoint: int = 50, n_gauges: int = 64):
+    """Adjoint of an n x n tidal basin (dx = 50 m, the tidal-basin bed
+    recipe and tide, still water, per-cell Manning n = 0.025, HLLC, CFL
+    0.45), `steps` steps with a checkpoint every `checkpoint`.  Loss: gauge
+    misfit at 64 seeded gauge cells over all steps, against the gauge series
+    of a run with the bed plus 0.5 m Gaussian noise and n = 0.03.  Cost of
+    the reverse sweep (segment replays included) against the plain forward
+    run, and central differences of the forward run along `fd_dirs` seeded
+    random directions in (b, n), with step 1e-6 and, because a step that
+    large crosses the scheme's minmod and wave-speed kinks somewhere in a
+    million cells over hundreds of steps, also with step 1e-8."""
+    import numpy as np
+
+    from rakali_amd import Grid
+    from rakali_amd import scenarios as S
+    from rakali_amd.adjoint import GaugeMisfit, start_still_water
+
+    scn = S.tidal_bumps(nx=n, ny=n, dx=50.0, steps=steps)
+    rng = np.random.default_rng(3)
+    b = scn.b
+    nf = np.full((n, n), 0.025)
+    gauges = [(int(rng.integers(0, n)), int(rng.integers(0, n))) for _ in range(n_gauges)]
+    kw = dict(scn.grid_kw)
+    kw.pop("manning_n")
+    g = Grid(n, n, scn.dx, scn.dy, manning_n=0.025, **kw)
+    sess = GaugeMisfit(g, steps, checkpoint, gauges)
+    # target: perturbed bed, n = 0.03
+    start_still_water(g, b + 0.5 * rng.normal(size=b.shape), np.full((n, n), 0.03))
+    sess.forward()
+    sess.set_targets(sess.gauge_eta())
+    # plain forward cost (same kernels: per-cell n)
+    start_still_water(g, b, nf)
+    g.step(2)
+    g.sync()
+    start_still_water(g, b, nf)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    fwd_s = time.perf_counter() - t0
+    # loss and gradients
+    start_still_water(g, b, nf)
+    t0 = time.perf_counter()
+    loss = sess.forward()
+    sess.backward()
+    sess.fold_still_water(0.0)
+    adj_s = time.perf_counter() - t0
+    f_ms, b_ms = sess.timing()
+    gb, gn = sess.gradient("b"), sess.gradient("n")
+    fd_steps = (1e-6, 1e-8)
+    errs = {e: [] for e in fd_steps}
+    drng = np.random.default_rng(11)
+    fd_t0 = time.perf_counter()
+    for _ in range(fd_dirs):
+        db = drng.normal(size=b.shape)
+        dn = drng.normal(size=b.shape) * 1e-3
+        ad = float(np.sum(gb * db) + np.sum(gn * dn))
+        for eps in fd_steps:
+            vals = []
+            for sgn in (1.0, -1.0):
+                start_still_water(g, b + sgn * eps * db, nf + sgn * eps * dn)
+                vals.append(sess.forward())
+            fd = (vals[0] - vals[1]) / (2 * eps)
+            errs[eps].append(abs(fd - ad) / max(abs(fd), 1e-300))
+    fd_s = time.perf_counter() - fd_t0
+    sess.close()
+    g.close()
+    return {
+        "workload": f"{n}x{n} tidal basin, dx=50 m, {steps} steps, checkpoint every {checkpoint}, "
+                    f"{n_gauges} gauges; target from bed + 0.5 m noise and n = 0.03; gradient w.r.t. "
+                    f"bed and per-cell Manning n",
+        "loss": loss,
+        "forward_ms_per_step": fwd_s / steps * 1e3,
+        "loss_forward_ms_per_step": f_ms / steps,
+        "reverse_ms_per_step": b_ms / steps,
+        "adjoint_over_forward": b_ms / (fwd_s * 1e3),
+        "forward_plus_adjoint_over_forward": adj_s / fwd_s,
+        "fd_directions": fd_dirs,
+        "fd_max_rel_err": max(errs[1e-6]),
+        "fd_median_rel_err": float(np.median(errs[1e-6])),
+        "fd_max_rel_err_step_1e-8": max(errs[1e-8]),
+        "fd_median_rel_err_step_1e-8": float(np.median(errs[1e-8])),
+        "fd_seconds": fd_s,
+        "max_abs_grad_b": float(np.abs(gb).max()),
+        "max_abs_grad_n": float(np.abs(gn).max()),
     }
 
 

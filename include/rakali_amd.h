@@ -66,6 +66,10 @@ extern "C" {
 #define RKG_FIELD_HS 7
 #define RKG_FIELD_HUS 8
 #define RKG_FIELD_HVS 9
+/* Per-cell Manning n (writable once rkg_use_manning_field has enabled it).
+ * The reference's manning_n is a scalar (rki_barotropic_state.F90:18-110);
+ * the field is an addition for gradients with respect to roughness. */
+#define RKG_FIELD_N 10
 
 #define RKG_MAX_TIDAL_CONSTITUENTS 10
 
@@ -142,6 +146,10 @@ int rkg_get_tile_rows(rkg_grid *g, int *rows);
 /* Keeps U^n of every step in three extra planes (24 B per cell per step of
  * extra traffic) so that RKG_FIELD_H0..HV0 can be read; off by default. */
 int rkg_set_rk_save(rkg_grid *g, int enable);
+/* enable != 0: friction uses a per-cell Manning n (field RKG_FIELD_N,
+ * initialised to the scalar manning_n; friction where n > 0); 0: back to
+ * the scalar.  No counterpart in the reference (scalar n only). */
+int rkg_use_manning_field(rkg_grid *g, int enable);
 /* Reads / replaces the scalar configuration (nx and ny cannot change):
  * boundary values, tide, friction, CFL, t_end, HLL/HLLC. */
 int rkg_get_config(rkg_grid *g, rkg_config *cfg);
@@ -200,6 +208,39 @@ int rkg_ensemble_get_schedule(rkg_ensemble *e, int *schedule, int *tile_rows, in
 /* Times `steps` steps of each candidate schedule and tile height on the
  * live ensemble, keeps the fastest, restores the state. */
 int rkg_ensemble_autotune(rkg_ensemble *e, int steps, double *best_ms_per_step);
+
+/* ------------------------------------------------------------------------
+ * Discrete adjoint of the step: gradients of a gauge misfit
+ *   L = sum_{steps s} sum_{gauges k} (h + b - target_{s,k})^2
+ * (h + b at the gauge cell after step s) with respect to the bed, the
+ * per-cell Manning n and the initial h, hu, hv.  No counterpart in the
+ * reference; for callers that fit bathymetry or roughness.  The forward
+ * run keeps a checkpoint every `checkpoint_interval` steps; the reverse
+ * sweep replays each segment from its checkpoint and reverses it.  dt and
+ * the boundary times are those of the forward run (exact at fixed dt).
+ * Creating an adjoint switches the grid to a per-cell Manning field.
+ * ------------------------------------------------------------------------ */
+typedef struct rkg_adjoint rkg_adjoint;     /* opaque handle */
+
+/* gauge cells (gauge_i[k], gauge_j[k]), 0-based interior indices;
+ * targets: n_steps x n_gauges, step-major (may be NULL: zeros). */
+int rkg_adjoint_create(rkg_grid *g, int n_steps, int checkpoint_interval, int n_gauges, const int *gauge_i,
+                       const int *gauge_j, const double *targets, rkg_adjoint **out);
+void rkg_adjoint_destroy(rkg_adjoint **a);
+int rkg_adjoint_set_targets(rkg_adjoint *a, const double *targets);
+/* n_steps steps from the grid's current state; returns L; the grid is
+ * left at the final state. */
+int rkg_adjoint_forward(rkg_adjoint *a, double *loss);
+/* gauge eta of every step of the last forward run (n_steps x n_gauges) */
+int rkg_adjoint_gauge_eta(rkg_adjoint *a, double *eta);
+/* gradients of loss_scale * L; tile_rows of the reverse face kernel (0 = 32) */
+int rkg_adjoint_backward(rkg_adjoint *a, double loss_scale, int tile_rows);
+/* adds the bed part of a still-water initial state h = max(eta0 - b, 0) */
+int rkg_adjoint_fold_still_water(rkg_adjoint *a, double eta0);
+/* field: RKG_FIELD_B, RKG_FIELD_N, or RKG_FIELD_H/HU/HV (initial state) */
+int rkg_adjoint_gradient(rkg_adjoint *a, int field, double *dst, int ld, int on_device);
+/* HIP-event time of the last forward and backward calls (ms) */
+int rkg_adjoint_timing(rkg_adjoint *a, double *forward_ms, double *backward_ms);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,7 @@ ERRORS = {-1: "not ready", -2: "argument out of range", -4: "HIP runtime error",
 BC = {"wall": 1, "open": 2, "tidal": 3, "inflow": 5, "discharge": 6,
       "clamped": 7, "sponge": 8}
 FIELDS = {"h": 0, "hu": 1, "hv": 2, "b": 3, "h0": 4, "hu0": 5, "hv0": 6,
-          "h_star": 7, "hu_star": 8, "hv_star": 9}
+          "h_star": 7, "hu_star": 8, "hv_star": 9, "n": 10}
 
 
 class Config(ctypes.Structure):
@@ -86,6 +86,7 @@ SIGNATURES = {
     "rkg_get_stream": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "rkg_copy_bandwidth": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp, _dp]),
     "rkg_set_rk_save": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_use_manning_field": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_get_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_set_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_ensemble_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, _P, ctypes.POINTER(_P)]),
@@ -101,6 +102,16 @@ SIGNATURES = {
     "rkg_ensemble_set_schedule": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "rkg_ensemble_get_schedule": (ctypes.c_int, [_P] + [ctypes.POINTER(ctypes.c_int)] * 4),
     "rkg_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_int, _dp]),
+    "rkg_adjoint_create": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P,
+                                          ctypes.POINTER(_P)]),
+    "rkg_adjoint_destroy": (None, [ctypes.POINTER(_P)]),
+    "rkg_adjoint_set_targets": (ctypes.c_int, [_P, _P]),
+    "rkg_adjoint_forward": (ctypes.c_int, [_P, _dp]),
+    "rkg_adjoint_gauge_eta": (ctypes.c_int, [_P, _P]),
+    "rkg_adjoint_backward": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int]),
+    "rkg_adjoint_fold_still_water": (ctypes.c_int, [_P, ctypes.c_double]),
+    "rkg_adjoint_gradient": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int]),
+    "rkg_adjoint_timing": (ctypes.c_int, [_P, _dp, _dp]),
 }
 
 

@@ -1,0 +1,146 @@
+"""Gradients of a gauge misfit through the GPU step, and their torch binding.
+
+``GaugeMisfit`` wraps ``rkg_adjoint_*`` (include/rakali_amd.h) on a ``Grid``:
+
+    L = sum_{steps s} sum_{gauges k} (h + b - target_{s,k})^2,
+
+with eta = h + b read at the gauge cells after every step.  ``forward``
+runs the steps from the grid's current state, keeping a checkpoint every
+``checkpoint_interval`` steps; ``backward`` replays each segment and
+reverses it, giving dL/db, dL/dn (per-cell Manning n) and dL/d(h, hu, hv)
+of the initial state.  The gradient is that of the discrete scheme with
+the forward run's dt and boundary times held fixed; with ``dt_fixed`` it is
+the exact gradient of the step.
+
+``BathymetryRoughnessLoss`` exposes the same computation as a
+``torch.autograd.Function`` of the bed and Manning fields, starting from
+still water.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELDS, check
+
+
+class GaugeMisfit:
+    def __init__(self, grid, n_steps: int, checkpoint_interval: int, gauges, targets=None):
+        self._L = _lib.load()
+        self.grid = grid
+        gi = np.ascontiguousarray([int(i) for i, _ in gauges], dtype=np.int32)
+        gj = np.ascontiguousarray([int(j) for _, j in gauges], dtype=np.int32)
+        self.n_steps, self.n_gauges = int(n_steps), len(gi)
+        t = None if targets is None else self._targets(targets)
+        self._h = ctypes.c_void_p()
+        check("rkg_adjoint_create", self._L.rkg_adjoint_create(
+            grid.handle, self.n_steps, int(checkpoint_interval), self.n_gauges,
+            gi.ctypes.data_as(ctypes.c_void_p), gj.ctypes.data_as(ctypes.c_void_p),
+            None if t is None else t.ctypes.data_as(ctypes.c_void_p), ctypes.byref(self._h)))
+
+    def _targets(self, targets) -> np.ndarray:
+        t = np.ascontiguousarray(targets, dtype=np.float64)
+        if t.shape != (self.n_steps, self.n_gauges):
+            raise ValueError(f"targets: expected shape {(self.n_steps, self.n_gauges)}, got {t.shape}")
+        return t
+
+    def set_targets(self, targets) -> None:
+        t = self._targets(targets)
+        check("rkg_adjoint_set_targets", self._L.rkg_adjoint_set_targets(self._h, t.ctypes.data_as(ctypes.c_void_p)))
+
+    def forward(self) -> float:
+        loss = ctypes.c_double()
+        check("rkg_adjoint_forward", self._L.rkg_adjoint_forward(self._h, ctypes.byref(loss)))
+        return loss.value
+
+    def gauge_eta(self) -> np.ndarray:
+        out = np.empty((self.n_steps, self.n_gauges), dtype=np.float64)
+        check("rkg_adjoint_gauge_eta", self._L.rkg_adjoint_gauge_eta(self._h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def backward(self, scale: float = 1.0, tile_rows: int = 0) -> None:
+        check("rkg_adjoint_backward", self._L.rkg_adjoint_backward(self._h, float(scale), int(tile_rows)))
+
+    def fold_still_water(self, eta0: float = 0.0) -> None:
+        check("rkg_adjoint_fold_still_water", self._L.rkg_adjoint_fold_still_water(self._h, float(eta0)))
+
+    def gradient(self, name: str) -> np.ndarray:
+        nx, ny = self.grid.nx, self.grid.ny
+        out = np.empty((nx, ny), dtype=np.float64, order="F")
+        check("rkg_adjoint_gradient", self._L.rkg_adjoint_gradient(
+            self._h, FIELDS[name], out.ctypes.data_as(ctypes.c_void_p), nx, 0))
+        return out
+
+    def timing(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        check("rkg_adjoint_timing", self._L.rkg_adjoint_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.rkg_adjoint_destroy(ctypes.byref(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def start_still_water(grid, b, n, eta0: float = 0.0) -> None:
+    """Bed `b`, Manning field `n`, still water h = max(eta0 - b, 0), at rest,
+    and the grid's clock back to t = 0."""
+    b = np.asfortranarray(b, dtype=np.float64)
+    grid.set_field("b", b)
+    grid.set_field("h", np.maximum(eta0 - b, 0.0))
+    z = np.zeros_like(b)
+    grid.set_field("hu", z)
+    grid.set_field("hv", z)
+    grid.use_manning_field(True)
+    grid.set_field("n", np.asfortranarray(n, dtype=np.float64))
+    grid.set_clock(0.0, 0)
+
+
+def _torch_function():
+    import torch
+
+    class BathymetryRoughnessLoss(torch.autograd.Function):
+        """loss = L(b, n): still water over bed b, per-cell Manning n,
+        ``session.n_steps`` steps, gauge misfit against the session's
+        targets.  Gradients come from the discrete adjoint."""
+
+        @staticmethod
+        def forward(ctx, b, n, session: GaugeMisfit, eta0: float = 0.0):
+            start_still_water(session.grid, b.detach().cpu().numpy(), n.detach().cpu().numpy(), eta0)
+            loss = session.forward()
+            ctx.session, ctx.eta0 = session, eta0
+            ctx.dev, ctx.dtype = b.device, b.dtype
+            return torch.tensor(loss, dtype=b.dtype, device=b.device)
+
+        @staticmethod
+        def backward(ctx, grad):
+            s = ctx.session
+            s.backward(scale=float(grad))
+            s.fold_still_water(ctx.eta0)
+            gb = torch.from_numpy(np.ascontiguousarray(s.gradient("b"))).to(device=ctx.dev, dtype=ctx.dtype)
+            gn = torch.from_numpy(np.ascontiguousarray(s.gradient("n"))).to(device=ctx.dev, dtype=ctx.dtype)
+            return gb, gn, None, None
+
+    return BathymetryRoughnessLoss
+
+
+def bathymetry_roughness_loss(b, n, session: GaugeMisfit, eta0: float = 0.0):
+    """torch entry point: ``loss = bathymetry_roughness_loss(b, n, session)``
+    with b, n float64 tensors of shape (nx, ny); ``loss.backward()`` fills
+    b.grad and n.grad from the adjoint."""
+    return _torch_function().apply(b, n, session, eta0)

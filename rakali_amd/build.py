@@ -20,8 +20,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "librakali_amd.so"
-SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip"]
-HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp",
+SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip", CSRC / "rkg_adjoint.hip"]
+HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp", CSRC / "adjoint.hpp",
            CSRC / "stage_kernels.hpp", CSRC / "rkg_internal.hpp", ROOT / "include" / "rakali_amd.h"]
 RKI_LIB = PKG / "librakali.so"
 RKI_SOURCES = [CSRC / "rki_api.cpp"]

@@ -1,0 +1,728 @@
+// rkg_adjoint.hip -- discrete adjoint of the SSP-RK2 step: gradients of a
+// gauge sea-surface misfit with respect to the bed, the per-cell Manning n
+// and the initial state.
+//
+// Loss: L = sum over steps s and gauges k of (eta_k(t_s) - target_{s,k})^2,
+// eta = h + b at the gauge cell after step s.  The forward pass runs the
+// ordinary stage kernels (per-cell Manning n) and keeps the state every
+// `checkpoint_interval` steps.  The reverse sweep takes segments last to
+// first: it replays a segment from its checkpoint with the recording stage
+// kernels, which keep U^n, U* and each stage's flux divergence for every
+// step of the segment, then reverses the steps one by one:
+//   k_adj_body  -- cell-local: RK2 average, clamp, KP07, Froude backstop,
+//                  Manning and Coriolis transposed; emits the adjoint of the
+//                  flux divergence of every cell and the Manning gradient;
+//   k_adj_faces -- tile kernel: every face and slope of the stage, with the
+//                  boundary ghosts, transposed onto the stage input and the
+//                  bed.
+// dt and the boundary times are those of the forward run (held fixed): the
+// gradient is exact for the discrete scheme at fixed time levels, and at
+// fixed dt (dt_fixed > 0) it is the exact gradient of the scheme.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rakali_amd.h"
+#include "adjoint.hpp"
+#include "rkg_internal.hpp"
+#include "stage_kernels.hpp"
+
+using namespace rk;
+
+namespace {
+
+constexpr unsigned ADJ_MAGIC = 0x72616B41u;  // 'rakA'
+constexpr unsigned ADJ_DEAD = 0xDEADBEEFu;
+constexpr int AW = 60;                        // own columns per tile of k_adj_faces
+
+// Own range [lo, hi) of block `s` when [0, n) is cut into blocks of `w`,
+// the last block holding at least two cells (a boundary ghost maps onto
+// the two cells next to the edge, which must belong to one tile).
+__host__ __device__ __forceinline__ void block_range(int n, int w, int s, int& lo, int& hi) {
+    const int nb = (n + w - 1) / w;
+    lo = s * w;
+    hi = min(lo + w, n);
+    if (nb >= 2 && n - (nb - 1) * w == 1) {
+        if (s == nb - 1) lo = n - 2;
+        if (s == nb - 2) hi = n - 2;
+    }
+}
+
+struct AdjStage {
+    GridDesc d;                 // geometry, physics, bed, n field; ah.. = stage input V
+    const double* un;           // U^n (3 planes), stage 2
+    const double* div;          // recorded flux divergence (3 planes)
+    double* lam_in;             // adjoint of the stage input (3 planes)
+    double* lam_out;            // adjoint of the stage output (3 planes); stage 2: the
+                                // adjoint of U^{n+1} in, replaced by its U^n part
+    double* lam_div;            // 3 planes
+    double* grad_b;
+    double* grad_n;
+    const double* hist;         // (t, dt) of the step
+    int ty;                     // rows per tile of k_adj_faces
+};
+
+__device__ __forceinline__ SourceTerms adj_src(const GridDesc& d, double dt) {
+    SourceTerms s;
+    s.g = d.g;
+    s.n_sq = d.manning_n * d.manning_n;
+    s.has_friction = d.manning_n > 0.0;
+    s.has_coriolis = fabs(d.coriolis_f) > 1.0e-20;
+    s.cosf = 0.0;
+    s.sinf = 0.0;
+    if (s.has_coriolis) {
+        s.cosf = crm::cos(d.coriolis_f * dt);
+        s.sinf = crm::sin(d.coriolis_f * dt);
+    }
+    return s;
+}
+
+// Cell-local reverse of one stage.  STAGE 2: lam_out holds the adjoint of
+// U^{n+1} and is replaced by the U^n part of the RK2 average; lam_in (the
+// adjoint of U*) is written.  STAGE 1: lam_out is the adjoint of U* and
+// lam_in (the adjoint of U^n) is added to.
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
+    const GridDesc& d = a.d;
+    const double dt = a.hist[1];
+    const SourceTerms src = adj_src(d, dt);
+    const long long P = d.plane;
+    const long long n = (long long)d.nx * d.ny;
+    for (long long idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+        const int j = (int)(idx / d.nx), i = (int)(idx % d.nx);
+        const long long k = (long long)j * d.pitch + i;
+        const double v0 = d.ah[k], v1 = d.ahu[k], v2 = d.ahv[k];
+        const adj::S3 div{a.div[k], a.div[P + k], a.div[2 * P + k]};
+        const double nc = d.nfield ? d.nfield[k] : d.manning_n;
+        const double nsq = d.nfield ? nc * nc : src.n_sq;
+        const bool fr = d.nfield ? (nc > 0.0) : (src.has_friction != 0);
+        adj::S3 lw;
+        if (STAGE == 2) {
+            double wh = v0, whu = v1, whv = v2;
+            adj::body_fwd(src, dt, div, nsq, fr, wh, whu, whv);
+            const double ah = 0.5 * (a.un[k] + wh), ahu = 0.5 * (a.un[P + k] + whu), ahv = 0.5 * (a.un[2 * P + k] + whv);
+            const adj::S3 lo{a.lam_out[k], a.lam_out[P + k], a.lam_out[2 * P + k]};
+            const adj::S3 la = adj::clamp_rev(d.g, ah, ahu, ahv, lo);
+            lw = adj::S3{0.5 * la.h, 0.5 * la.hu, 0.5 * la.hv};
+            a.lam_out[k] = lw.h; a.lam_out[P + k] = lw.hu; a.lam_out[2 * P + k] = lw.hv;
+        } else {
+            lw = adj::S3{a.lam_out[k], a.lam_out[P + k], a.lam_out[2 * P + k]};
+        }
+        const adj::BodyRev r = adj::body_rev(src, dt, div, nc, nsq, fr, v0, v1, v2, lw);
+        if (STAGE == 2) {
+            a.lam_in[k] = r.lv.h; a.lam_in[P + k] = r.lv.hu; a.lam_in[2 * P + k] = r.lv.hv;
+        } else {
+            a.lam_in[k] += r.lv.h; a.lam_in[P + k] += r.lv.hu; a.lam_in[2 * P + k] += r.lv.hv;
+        }
+        a.lam_div[k] = r.ldiv.h; a.lam_div[P + k] = r.ldiv.hu; a.lam_div[2 * P + k] = r.ldiv.hv;
+        if (d.nfield && a.grad_n) a.grad_n[k] += r.ln;
+    }
+}
+
+struct Acc {   // adjoint accumulator of one cell: (h, hu, hv, b)
+    double h, hu, hv, b;
+};
+
+__device__ __forceinline__ void add(Acc& a, const adj::C4& c) { a.h += c.h; a.hu += c.hu; a.hv += c.hv; a.b += c.b; }
+__device__ __forceinline__ adj::C4 c4(const Acc& a) { return adj::C4{a.h, a.hu, a.hv, a.b}; }
+__device__ __forceinline__ double sh_dn(double v) { return __shfl_down(v, 1, WAVE); }
+__device__ __forceinline__ double sh_up(double v) { return __shfl_up(v, 1, WAVE); }
+__device__ __forceinline__ adj::C4 sh_dn(const adj::C4& c) { return adj::C4{sh_dn(c.h), sh_dn(c.hu), sh_dn(c.hv), sh_dn(c.b)}; }
+__device__ __forceinline__ adj::C4 sh_up(const adj::C4& c) { return adj::C4{sh_up(c.h), sh_up(c.hu), sh_up(c.hv), sh_up(c.b)}; }
+__device__ __forceinline__ adj::C4 sh_from(const adj::C4& c, int src) {
+    return adj::C4{__shfl(c.h, src, WAVE), __shfl(c.hu, src, WAVE), __shfl(c.hv, src, WAVE), __shfl(c.b, src, WAVE)};
+}
+
+__device__ __forceinline__ adj::S3 load_ldiv(const AdjStage& a, int x, int y) {
+    const GridDesc& d = a.d;
+    if ((unsigned)x >= (unsigned)d.nx || (unsigned)y >= (unsigned)d.ny) return adj::S3{0.0, 0.0, 0.0};
+    const long long k = (long long)y * d.pitch + x;
+    return adj::S3{a.lam_div[k], a.lam_div[d.plane + k], a.lam_div[2 * d.plane + k]};
+}
+
+__device__ __forceinline__ Cell vload(const GridDesc& d, int x, int y, double eta) {
+    return load_cell<true>(d, d.ah, d.ahu, d.ahv, x, y, eta);
+}
+
+// Adjoint of the row-y ghost at column x onto interior cells (x-direction
+// ghosts: normal momentum hu).  Returns the contributions for the mirror
+// and first-interior columns.
+__device__ __forceinline__ void ghost_x_rev(const GridDesc& d, double eta, int x, int y, const adj::C4& lg, int& mcol,
+                                            int& icol, adj::C4& lm, adj::C4& li) {
+    const bool west = x < 0;
+    const int kk = west ? (-x - 1) : (x - d.nx);
+    mcol = west ? kk : (d.nx - 1 - kk);
+    icol = west ? 0 : (d.nx - 1);
+    const Cell mir = load_direct(d.ah, d.ahu, d.ahv, d.bed, d.pitch, mcol, y);
+    const Cell in0 = load_direct(d.ah, d.ahu, d.ahv, d.bed, d.pitch, icol, y);
+    lm = adj::C4{0, 0, 0, 0};
+    li = adj::C4{0, 0, 0, 0};
+    adj::ghost_rev(d.bc[west ? 0 : 1], d.g, eta, d.inflow_velocity, d.clamped_elevation, mir, in0, west ? -1.0 : 1.0,
+                   true, lg, lm, li);
+}
+
+__device__ __forceinline__ void ghost_y_rev(const GridDesc& d, double eta, int x, int y, const adj::C4& lg, int& mrow,
+                                            int& irow, adj::C4& lm, adj::C4& li) {
+    const bool south = y < 0;
+    const int kk = south ? (-y - 1) : (y - d.ny);
+    mrow = south ? kk : (d.ny - 1 - kk);
+    irow = south ? 0 : (d.ny - 1);
+    const Cell mir = load_direct(d.ah, d.ahu, d.ahv, d.bed, d.pitch, x, mrow);
+    const Cell in0 = load_direct(d.ah, d.ahu, d.ahv, d.bed, d.pitch, x, irow);
+    lm = adj::C4{0, 0, 0, 0};
+    li = adj::C4{0, 0, 0, 0};
+    adj::ghost_rev(d.bc[south ? 2 : 3], d.g, eta, d.inflow_velocity, d.clamped_elevation, mir, in0,
+                   south ? -1.0 : 1.0, false, lg, lm, li);
+}
+
+constexpr int NWIN = 6;   // accumulator rows r-3 .. r+2
+
+// Reverse of all faces and slopes of one stage onto the stage input and
+// the bed.  One wavefront per tile; lane l holds column c0 - 2 + l, the
+// tile owns columns [c0, c1) (at most 60) and rows [j0, j1).  The rows are
+// marched from j0 - 2 to j1: iteration r reverses the y face between rows
+// r and r+1, the y slope of row r and, for an owned row, its x faces and
+// x slopes.  Row r-1 is complete after iteration r; rows are written two
+// iterations later so that ghost rows can still be folded into them.
+template <bool HLLC, int STAGE>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_faces(AdjStage a) {
+    const GridDesc& d = a.d;
+    const double tau = STAGE == 1 ? a.hist[0] : a.hist[0] + a.hist[1];   // bc%time of the stage
+    const int lane = threadIdx.x;
+    const int nstr = (d.nx + AW - 1) / AW;
+    const int sx = blockIdx.x % nstr, sy = blockIdx.x / nstr;
+    int c0, c1, j0, j1;
+    block_range(d.nx, AW, sx, c0, c1);
+    block_range(d.ny, a.ty, sy, j0, j1);
+    const int x = c0 - 2 + lane;
+    const bool xin = (unsigned)x < (unsigned)d.nx;
+    const bool own_col = x >= c0 && x < c1;
+    const double eta = tidal_eta(d, tau);
+    const double g = d.g;
+    const double rdx = 1.0 / d.dx, rdy = 1.0 / d.dy;
+    const long long P = d.plane;
+
+    Acc acc[NWIN];
+#pragma unroll
+    for (int q = 0; q < NWIN; ++q) acc[q] = Acc{0.0, 0.0, 0.0, 0.0};
+    const int rs = j0 - 2, re = j1;
+    Cell v0 = vload(d, x, rs - 1, eta), v1 = vload(d, x, rs, eta), v2 = vload(d, x, rs + 1, eta), v3;
+    FState hi_r, lo_tmp;
+    adj::states_fwd(v0, v1, v2, lo_tmp, hi_r);
+    adj::S3 llo_r{0.0, 0.0, 0.0};        // adjoint of the y lo-state of row r (from face r - 1/2)
+    adj::S3 ld_r = load_ldiv(a, x, rs);
+
+    // Folds the completed row `rho` (window slot `q`) into its targets:
+    // ghost rows onto their source rows, ghost columns onto their source
+    // columns, owned cells into the adjoint arrays.
+    auto flush = [&](int rho, int q, int r) {
+        if (rho < -2 || rho > d.ny + 1) return;
+        const adj::C4 lg = c4(acc[q]);
+        if (rho < 0 || rho >= d.ny) {
+            if (xin) {
+                int mrow, irow;
+                adj::C4 lm, li;
+                ghost_y_rev(d, eta, x, rho, lg, mrow, irow, lm, li);
+                // source rows are in the window: slot of row t is t - (r - 3)
+                const int qm = mrow - (r - 3), qi = irow - (r - 3);
+#pragma unroll
+                for (int s = 0; s < NWIN; ++s) {
+                    if (s == qm) add(acc[s], lm);
+                    if (s == qi) add(acc[s], li);
+                }
+            }
+            return;
+        }
+        // ghost columns of an owned row fold onto their source columns
+        // (tiles whose lanes reach past an edge only)
+        adj::C4 mine = lg;
+        if (c0 - 2 < 0 || c0 + WAVE - 3 >= d.nx) {
+            adj::C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
+            int mcol = -100, icol = -100;
+            if (x == -1 || x == -2 || x == d.nx || x == d.nx + 1) ghost_x_rev(d, eta, x, rho, lg, mcol, icol, lm, li);
+            // lanes holding columns -2, -1, nx, nx + 1 (when inside the wavefront)
+            const int srcs[4] = {-c0, 1 - c0, d.nx - c0 + 2, d.nx - c0 + 3};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int src = srcs[s];
+                if (src < 0 || src >= WAVE) continue;
+                const adj::C4 rm = sh_from(lm, src), ri = sh_from(li, src);
+                const int rmcol = __shfl(mcol, src, WAVE), ricol = __shfl(icol, src, WAVE);
+                if (x == rmcol) { mine.h += rm.h; mine.hu += rm.hu; mine.hv += rm.hv; mine.b += rm.b; }
+                if (x == ricol) { mine.h += ri.h; mine.hu += ri.hu; mine.hv += ri.hv; mine.b += ri.b; }
+            }
+        }
+        if (own_col && rho >= j0 && rho < j1) {
+            const long long k = (long long)rho * d.pitch + x;
+            a.lam_in[k] += mine.h;
+            a.lam_in[P + k] += mine.hu;
+            a.lam_in[2 * P + k] += mine.hv;
+            a.grad_b[k] += mine.b;
+        }
+    };
+
+    for (int r = rs; r <= re; ++r) {
+        // window: acc[q] is row r - 3 + q; v0..v3 rows r-1 .. r+2
+        v3 = vload(d, x, r + 2, eta);
+        const adj::S3 ld_r1 = load_ldiv(a, x, r + 1);
+        FState lo_r1, hi_r1;
+        adj::states_fwd(v1, v2, v3, lo_r1, hi_r1);
+        // y face r + 1/2 (interior columns, faces -1/2 .. ny - 1/2)
+        adj::S3 lhi_r{0.0, 0.0, 0.0}, llo_r1{0.0, 0.0, 0.0};
+        if (xin && r >= -1 && r <= d.ny - 1) {
+            const Face lf = adj::face_seed<true>(g, rdy, ld_r, ld_r1);
+            double lbL = 0.0, lbR = 0.0;
+            adj::face_rev<HLLC, true>(g, hi_r, v1.b, lo_r1, v2.b, lf, lhi_r, lbL, llo_r1, lbR);
+            acc[3].b += lbL;
+            acc[4].b += lbR;
+        }
+        // y slope of row r
+        {
+            adj::C4 lm{0, 0, 0, 0}, lc{0, 0, 0, 0}, lp{0, 0, 0, 0};
+            adj::states_rev(v0, v1, v2, llo_r, lhi_r, lm, lc, lp);
+            add(acc[2], lm);
+            add(acc[3], lc);
+            add(acc[4], lp);
+        }
+        // x faces and slopes of an owned row
+        if (r >= j0 && r < j1) {
+            Cell xm, xp;
+            xm.h = sh_up(v1.h); xm.hu = sh_up(v1.hu); xm.hv = sh_up(v1.hv); xm.b = sh_up(v1.b);
+            xp.h = sh_dn(v1.h); xp.hu = sh_dn(v1.hu); xp.hv = sh_dn(v1.hv); xp.b = sh_dn(v1.b);
+            if (lane == 0) xm = vload(d, x - 1, r, eta);
+            if (lane == WAVE - 1) xp = vload(d, x + 1, r, eta);
+            FState lo_x, hi_x;
+            adj::states_fwd(xm, v1, xp, lo_x, hi_x);
+            FState R;
+            R.h = sh_dn(lo_x.h); R.hu = sh_dn(lo_x.hu); R.hv = sh_dn(lo_x.hv);
+            adj::S3 ld_next{sh_dn(ld_r.h), sh_dn(ld_r.hu), sh_dn(ld_r.hv)};
+            adj::S3 lhi_x{0.0, 0.0, 0.0}, lR{0.0, 0.0, 0.0};
+            double lbL = 0.0, lbR = 0.0;
+            if (lane < WAVE - 1 && x >= -1 && x <= d.nx - 1) {
+                const Face lf = adj::face_seed<false>(g, rdx, ld_r, ld_next);
+                adj::face_rev<HLLC, false>(g, hi_x, v1.b, R, xp.b, lf, lhi_x, lbL, lR, lbR);
+            }
+            adj::S3 llo_x{sh_up(lR.h), sh_up(lR.hu), sh_up(lR.hv)};
+            const double lb_from_left = sh_up(lbR);
+            if (lane == 0) { llo_x = adj::S3{0.0, 0.0, 0.0}; }
+            acc[3].b += lbL + (lane == 0 ? 0.0 : lb_from_left);
+            adj::C4 lm{0, 0, 0, 0}, lc{0, 0, 0, 0}, lp{0, 0, 0, 0};
+            adj::states_rev(xm, v1, xp, llo_x, lhi_x, lm, lc, lp);
+            const adj::C4 from_right = sh_dn(lm);   // lane l+1's left-neighbour part
+            const adj::C4 from_left = sh_up(lp);    // lane l-1's right-neighbour part
+            add(acc[3], lc);
+            if (lane < WAVE - 1) add(acc[3], from_right);
+            if (lane > 0) add(acc[3], from_left);
+        }
+        // row r - 3 is complete and past the ghost folding window
+        flush(r - 3, 0, r);
+        // shift
+#pragma unroll
+        for (int q = 0; q < NWIN - 1; ++q) acc[q] = acc[q + 1];
+        acc[NWIN - 1] = Acc{0.0, 0.0, 0.0, 0.0};
+        v0 = v1; v1 = v2; v2 = v3;
+        hi_r = hi_r1;
+        llo_r = llo_r1;
+        ld_r = ld_r1;
+    }
+    // remaining rows re-2 .. re+2 in slots 0..4 (window base re + 1 - 3):
+    // ghost rows first (their sources are among the owned rows), then the rest
+    const int base = re + 1 - 3;
+    for (int q = 0; q < NWIN; ++q) {
+        const int rho = base + q;
+        if (rho >= d.ny) flush(rho, q, re + 1);
+    }
+    for (int q = 0; q < NWIN; ++q) {
+        const int rho = base + q;
+        if (rho < d.ny) flush(rho, q, re + 1);
+    }
+}
+
+// Gauge misfit of one state: loss += sum_k (h + b - target_k)^2, eta
+// recorded per gauge; a single workgroup, fixed summation order.
+__global__ __launch_bounds__(256) void k_gauge_loss(const double* __restrict__ h, const double* __restrict__ bed,
+                                                   long long pitch, const int* __restrict__ gi,
+                                                   const int* __restrict__ gj, const double* __restrict__ tgt,
+                                                   int ng, double* __restrict__ eta_out, double* __restrict__ loss) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < ng; k += 256) {
+        const long long o = (long long)gj[k] * pitch + gi[k];
+        const double e = h[o] + bed[o];
+        if (eta_out) eta_out[k] = e;
+        if (tgt) {
+            const double m = e - tgt[k];
+            acc = acc + m * m;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = *loss + red[0];
+}
+
+// Seeds d L / d U^{n+1} at the gauges (one thread: gauges may share cells).
+__global__ void k_gauge_seed(const double* __restrict__ h, const double* __restrict__ bed, long long pitch,
+                             const int* __restrict__ gi, const int* __restrict__ gj, const double* __restrict__ tgt,
+                             int ng, double scale, double* __restrict__ lam_h, double* __restrict__ grad_b) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = 0; k < ng; ++k) {
+        const long long o = (long long)gj[k] * pitch + gi[k];
+        const double s = 2.0 * scale * ((h[o] + bed[o]) - tgt[k]);
+        lam_h[o] += s;
+        grad_b[o] += s;
+    }
+}
+
+// (t, dt) of the step that starts from control slot p.
+__global__ void k_hist(const CtrlSlot* ctrl, int p, double* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out[0] = ctrl[p].t;
+        out[1] = ctrl[1 - p].dt;
+    }
+}
+
+// Still-water initial state: h = max(eta0 - b, 0) and its bed adjoint.
+__global__ __launch_bounds__(256) void k_fold_still_water(const double* __restrict__ bed, const double* __restrict__ lam_h,
+                                                         double eta0, int nx, int ny, long long pitch,
+                                                         double* __restrict__ grad_b) {
+    const long long n = (long long)nx * ny;
+    for (long long idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+        const long long k = (idx / nx) * pitch + idx % nx;
+        if (eta0 - bed[k] > 0.0) grad_b[k] -= lam_h[k];
+    }
+}
+
+}  // namespace
+
+struct rkg_adjoint {
+    unsigned magic;
+    rkg_grid* g;
+    int nsteps, K, nseg, ng;
+    int* gi;                    // device
+    int* gj;
+    double* tgt;                // device, nsteps x ng
+    double* eta;                // device, nsteps x ng (gauge eta of the last forward run)
+    double* ckpt;               // nseg x 3 planes
+    CtrlSlot* ckpt_ctrl;        // nseg x 2 (device)
+    std::vector<int> ckpt_parity;
+    double* ws;                 // segment workspace: A (K+1) x 3, B K x 3, div1 K x 3, div2 K x 3
+    double* lam;                // lamA (3), lamB (3), lamDiv (3), grad_b, grad_n
+    double* hist;               // K x 2
+    double* loss_dev;
+    double* loss_host;
+    CtrlSlot end_ctrl[2];
+    int end_parity;
+    bool have_forward;
+    float fwd_ms, bwd_ms;
+    hipEvent_t ev[2];
+};
+
+namespace {
+
+bool valid(const rkg_adjoint* a) { return a != nullptr && a->magic == ADJ_MAGIC; }
+
+size_t P(const rkg_adjoint* a) { return a->g->plane; }
+double* wsA(rkg_adjoint* a, int k) { return a->ws + (size_t)k * 3 * P(a); }
+double* wsB(rkg_adjoint* a, int k) { return a->ws + (size_t)(a->K + 1 + k) * 3 * P(a); }
+double* wsD1(rkg_adjoint* a, int k) { return a->ws + (size_t)(2 * a->K + 1 + k) * 3 * P(a); }
+double* wsD2(rkg_adjoint* a, int k) { return a->ws + (size_t)(3 * a->K + 1 + k) * 3 * P(a); }
+double* lamA(rkg_adjoint* a) { return a->lam; }
+double* lamB(rkg_adjoint* a) { return a->lam + 3 * P(a); }
+double* lamDiv(rkg_adjoint* a) { return a->lam + 6 * P(a); }
+double* gradB(rkg_adjoint* a) { return a->lam + 9 * P(a); }
+double* gradN(rkg_adjoint* a) { return a->lam + 10 * P(a); }
+
+GridDesc stage_desc(rkg_adjoint* a, double* in, double* in2, double* out2) {
+    GridDesc d = make_desc(a->g);
+    d.ah = in; d.ahu = in + P(a); d.ahv = in + 2 * P(a);
+    d.bh = in2; d.bhu = in2 + P(a); d.bhv = in2 + 2 * P(a);
+    d.o2h = out2; d.o2hu = out2 + P(a); d.o2hv = out2 + 2 * P(a);
+    d.sh = d.shu = d.shv = nullptr;
+    return d;
+}
+
+template <int STAGE>
+void launch_rec(rkg_grid* g, const GridDesc& d, int p) {
+    const int ntiles = d.nstrips * d.nsegs;
+    const double tlim = g->cfg.t_end;
+    if (g->cfg.use_hllc)
+        hipLaunchKernelGGL((k_stage<STAGE, true, true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, p, MODE_STEP, tlim);
+    else
+        hipLaunchKernelGGL((k_stage<STAGE, false, true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, p, MODE_STEP, tlim);
+}
+
+int body_blocks(const rkg_grid* g) {
+    const long long n = (long long)g->cfg.nx * g->cfg.ny;
+    return (int)std::min<long long>((n + 255) / 256, 8192);
+}
+
+int faces_tiles(const rkg_grid* g, int ty) {
+    const int nstr = (g->cfg.nx + AW - 1) / AW;
+    const int nseg = (g->cfg.ny + ty - 1) / ty;
+    return nstr * nseg;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_i, const int* gauge_j,
+                       const double* targets, rkg_adjoint** out) {
+    if (!out) { set_error("null argument"); return RKG_ERR_RANGE; }
+    *out = nullptr;
+    if (!g || g->magic != 0x72616B47u) return RKG_ERR_BAD_HANDLE;
+    if (nsteps < 1 || K < 1 || ng < 0 || (ng > 0 && (!gauge_i || !gauge_j))) {
+        set_error("need nsteps >= 1, checkpoint_interval >= 1 and gauge indices");
+        return RKG_ERR_RANGE;
+    }
+    for (int k = 0; k < ng; ++k)
+        if (gauge_i[k] < 0 || gauge_i[k] >= g->cfg.nx || gauge_j[k] < 0 || gauge_j[k] >= g->cfg.ny) {
+            set_error("gauge %d at (%d, %d) lies outside the %dx%d grid", k, gauge_i[k], gauge_j[k], g->cfg.nx,
+                      g->cfg.ny);
+            return RKG_ERR_RANGE;
+        }
+    int rc = rkg_use_manning_field(g, 1);
+    if (rc) return rc;
+    rkg_adjoint* a = new rkg_adjoint();
+    memset((void*)a, 0, sizeof *a);
+    a->ckpt_parity = std::vector<int>();
+    a->g = g;
+    a->nsteps = nsteps;
+    a->K = std::min(K, nsteps);
+    a->nseg = (nsteps + a->K - 1) / a->K;
+    a->ng = ng;
+    a->ckpt_parity.assign(a->nseg, 0);
+    const size_t Pl = g->plane;
+    hipError_t e = hipSuccess;
+    if (ng > 0) {
+        e = hipMalloc((void**)&a->gi, ng * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&a->gj, ng * sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(a->gi, gauge_i, ng * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(a->gj, gauge_j, ng * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc((void**)&a->tgt, (size_t)nsteps * ng * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void**)&a->eta, (size_t)nsteps * ng * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(a->tgt, 0, (size_t)nsteps * ng * sizeof(double));
+        if (e == hipSuccess && targets)
+            e = hipMemcpy(a->tgt, targets, (size_t)nsteps * ng * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&a->ckpt, (size_t)a->nseg * 3 * Pl * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->ckpt_ctrl, (size_t)a->nseg * 2 * sizeof(CtrlSlot));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->ws, (size_t)(12 * a->K + 3) * Pl * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->lam, 11 * Pl * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->hist, (size_t)a->K * 2 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->loss_dev, sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&a->loss_host, sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemset(a->ws, 0, (size_t)(12 * a->K + 3) * Pl * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(a->lam, 0, 11 * Pl * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&a->ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&a->ev[1]);
+    a->magic = ADJ_MAGIC;
+    if (e != hipSuccess) {
+        set_error("adjoint allocation failed: %s", hipGetErrorString(e));
+        rkg_adjoint_destroy(&a);
+        return RKG_ERR_HIP;
+    }
+    *out = a;
+    return RKG_OK;
+}
+
+void rkg_adjoint_destroy(rkg_adjoint** pa) {
+    if (!pa || !*pa) return;
+    rkg_adjoint* a = *pa;
+    if (a->magic != ADJ_MAGIC) { *pa = nullptr; return; }
+    if (a->g && a->g->stream) (void)hipStreamSynchronize(a->g->stream);
+    for (void* p : {(void*)a->gi, (void*)a->gj, (void*)a->tgt, (void*)a->eta, (void*)a->ckpt, (void*)a->ckpt_ctrl,
+                    (void*)a->ws, (void*)a->lam, (void*)a->hist, (void*)a->loss_dev})
+        if (p) (void)hipFree(p);
+    if (a->loss_host) (void)hipHostFree(a->loss_host);
+    if (a->ev[0]) (void)hipEventDestroy(a->ev[0]);
+    if (a->ev[1]) (void)hipEventDestroy(a->ev[1]);
+    a->magic = ADJ_DEAD;
+    delete a;
+    *pa = nullptr;
+}
+
+int rkg_adjoint_set_targets(rkg_adjoint* a, const double* targets) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    if (a->ng == 0) return RKG_OK;
+    if (!targets) return RKG_ERR_RANGE;
+    HIP_TRY(hipMemcpy(a->tgt, targets, (size_t)a->nsteps * a->ng * sizeof(double), hipMemcpyHostToDevice));
+    return RKG_OK;
+}
+
+int rkg_adjoint_gauge_eta(rkg_adjoint* a, double* eta) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    if (!a->have_forward) { set_error("no forward run yet"); return RKG_ERR_NOT_READY; }
+    if (a->ng == 0) return RKG_OK;
+    if (!eta) return RKG_ERR_RANGE;
+    HIP_TRY(hipMemcpy(eta, a->eta, (size_t)a->nsteps * a->ng * sizeof(double), hipMemcpyDeviceToHost));
+    return RKG_OK;
+}
+
+// nsteps steps from the grid's current state, recording checkpoints and the
+// gauge misfit; the grid is left at the final state.
+int rkg_adjoint_forward(rkg_adjoint* a, double* loss) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    rkg_grid* g = a->g;
+    const size_t Pl = g->plane;
+    HIP_TRY(hipMemsetAsync(a->loss_dev, 0, sizeof(double), g->stream));
+    HIP_TRY(hipEventRecord(a->ev[0], g->stream));
+    for (int n = 0; n < a->nsteps; ++n) {
+        if (n % a->K == 0) {
+            const int s = n / a->K;
+            if (g->speed_stale) {   // make the control slot current before saving it
+                int rc = rkg_step(g, 0);
+                if (rc) return rc;
+            }
+            HIP_TRY(hipMemcpyAsync(a->ckpt + (size_t)s * 3 * Pl, g->buf, 3 * Pl * sizeof(double),
+                                   hipMemcpyDeviceToDevice, g->stream));
+            HIP_TRY(hipMemcpyAsync(a->ckpt_ctrl + 2 * s, g->ctrl, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice,
+                                   g->stream));
+            a->ckpt_parity[s] = g->parity;
+        }
+        int rc = rkg_step(g, 1);
+        if (rc) return rc;
+        if (a->ng > 0)
+            hipLaunchKernelGGL(k_gauge_loss, dim3(1), dim3(256), 0, g->stream, g->buf, g->buf + 6 * Pl,
+                               (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng, a->ng,
+                               a->eta + (size_t)n * a->ng, a->loss_dev);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(a->ev[1], g->stream));
+    HIP_TRY(hipMemcpyAsync(a->end_ctrl, g->ctrl, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(a->loss_host, a->loss_dev, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
+    a->end_parity = g->parity;
+    a->have_forward = true;
+    if (loss) *loss = *a->loss_host;
+    return RKG_OK;
+}
+
+// Reverse sweep: gradients of loss_scale * L with respect to the bed, the
+// per-cell Manning n and the initial state (h, hu, hv).  The grid's state
+// is left as the forward run left it.
+int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    if (!a->have_forward) { set_error("rkg_adjoint_backward before rkg_adjoint_forward"); return RKG_ERR_NOT_READY; }
+    if (tile_rows <= 0) tile_rows = 32;
+    if (tile_rows < 4 || tile_rows > 256) { set_error("tile_rows must lie in [4, 256]"); return RKG_ERR_RANGE; }
+    rkg_grid* g = a->g;
+    const size_t Pl = g->plane;
+    HIP_TRY(hipEventRecord(a->ev[0], g->stream));
+    HIP_TRY(hipMemsetAsync(a->lam, 0, 11 * Pl * sizeof(double), g->stream));
+    const int fblocks = faces_tiles(g, tile_rows);
+    for (int s = a->nseg - 1; s >= 0; --s) {
+        const int n0 = s * a->K, n1 = std::min(a->nsteps, n0 + a->K), Ks = n1 - n0;
+        // replay the segment with recording
+        HIP_TRY(hipMemcpyAsync(wsA(a, 0), a->ckpt + (size_t)s * 3 * Pl, 3 * Pl * sizeof(double),
+                               hipMemcpyDeviceToDevice, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->ctrl, a->ckpt_ctrl + 2 * s, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice,
+                               g->stream));
+        int p = a->ckpt_parity[s];
+        for (int k = 0; k < Ks; ++k) {
+            GridDesc d1 = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
+            d1.rec = wsD1(a, k);
+            launch_rec<1>(g, d1, p);
+            // after stage 1: slot p holds t of the step, slot 1 - p its dt
+            hipLaunchKernelGGL(k_hist, dim3(1), dim3(64), 0, g->stream, g->ctrl, p, a->hist + 2 * k);
+            GridDesc d2 = d1;
+            d2.rec = wsD2(a, k);
+            launch_rec<2>(g, d2, p);
+            p ^= 1;
+        }
+        // reverse the steps of the segment
+        for (int k = Ks - 1; k >= 0; --k) {
+            const int n = n0 + k;
+            if (a->ng > 0)
+                hipLaunchKernelGGL(k_gauge_seed, dim3(1), dim3(64), 0, g->stream, wsA(a, k + 1), g->buf + 6 * Pl,
+                                   (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng, a->ng, loss_scale,
+                                   lamA(a), gradB(a));
+            AdjStage st;
+            // stage 2: input U* (B_k), U^n = A_k
+            st.d = stage_desc(a, wsB(a, k), wsB(a, k), wsA(a, k + 1));
+            st.un = wsA(a, k);
+            st.div = wsD2(a, k);
+            st.lam_in = lamB(a);
+            st.lam_out = lamA(a);
+            st.lam_div = lamDiv(a);
+            st.grad_b = gradB(a);
+            st.grad_n = gradN(a);
+            st.hist = a->hist + 2 * k;
+            st.ty = tile_rows;
+            hipLaunchKernelGGL(k_adj_body<2>, dim3(body_blocks(g)), dim3(256), 0, g->stream, st);
+            if (g->cfg.use_hllc)
+                hipLaunchKernelGGL((k_adj_faces<true, 2>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
+            else
+                hipLaunchKernelGGL((k_adj_faces<false, 2>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
+            // stage 1: input U^n (A_k)
+            st.d = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
+            st.un = nullptr;
+            st.div = wsD1(a, k);
+            st.lam_in = lamA(a);
+            st.lam_out = lamB(a);
+            hipLaunchKernelGGL(k_adj_body<1>, dim3(body_blocks(g)), dim3(256), 0, g->stream, st);
+            if (g->cfg.use_hllc)
+                hipLaunchKernelGGL((k_adj_faces<true, 1>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
+            else
+                hipLaunchKernelGGL((k_adj_faces<false, 1>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    // the grid's control slots back to the end of the forward run
+    HIP_TRY(hipMemcpyAsync(g->ctrl, a->end_ctrl, 2 * sizeof(CtrlSlot), hipMemcpyHostToDevice, g->stream));
+    g->parity = a->end_parity;
+    HIP_TRY(hipEventRecord(a->ev[1], g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipEventElapsedTime(&a->bwd_ms, a->ev[0], a->ev[1]));
+    return RKG_OK;
+}
+
+// Adds the adjoint of a still-water initial state (h = max(eta0 - b, 0),
+// hu = hv = 0) to the bed gradient.
+int rkg_adjoint_fold_still_water(rkg_adjoint* a, double eta0) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    rkg_grid* g = a->g;
+    hipLaunchKernelGGL(k_fold_still_water, dim3(body_blocks(g)), dim3(256), 0, g->stream, g->buf + 6 * g->plane,
+                       lamA(a), eta0, g->cfg.nx, g->cfg.ny, (long long)g->pitch, gradB(a));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return RKG_OK;
+}
+
+// field: RKG_FIELD_B (bed), RKG_FIELD_N (Manning n), RKG_FIELD_H/HU/HV
+// (initial state).  on_device != 0: dst is device memory.
+int rkg_adjoint_gradient(rkg_adjoint* a, int field, double* dst, int ld, int on_device) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    rkg_grid* g = a->g;
+    const double* src;
+    switch (field) {
+    case RKG_FIELD_H: src = lamA(a); break;
+    case RKG_FIELD_HU: src = lamA(a) + g->plane; break;
+    case RKG_FIELD_HV: src = lamA(a) + 2 * g->plane; break;
+    case RKG_FIELD_B: src = gradB(a); break;
+    case RKG_FIELD_N: src = gradN(a); break;
+    default: set_error("no gradient for field %d", field); return RKG_ERR_RANGE;
+    }
+    if (!dst || ld < g->cfg.nx) { set_error("bad buffer or leading dimension"); return RKG_ERR_RANGE; }
+    HIP_TRY(hipMemcpy2DAsync(dst, (size_t)ld * sizeof(double), src, (size_t)g->pitch * sizeof(double),
+                             (size_t)g->cfg.nx * sizeof(double), (size_t)g->cfg.ny,
+                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return RKG_OK;
+}
+
+int rkg_adjoint_timing(rkg_adjoint* a, double* forward_ms, double* backward_ms) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    if (forward_ms) *forward_ms = a->fwd_ms;
+    if (backward_ms) *backward_ms = a->bwd_ms;
+    return RKG_OK;
+}
+
+}  // extern "C"

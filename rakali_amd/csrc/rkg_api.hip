@@ -59,81 +59,10 @@ __global__ void k_math_eval(int fn, const double* x, double* y, int n) {
 
 }  // namespace
 
-struct rkg_grid {
-    unsigned magic;
-    rkg_config cfg;
-    int device;
-    hipStream_t stream;
-    bool own_stream;
-    long long pitch;
-    size_t plane;        // elements per field plane
-    double* buf;         // 7 planes: A(h,hu,hv) B(h,hu,hv) bed
-    double* save;        // 3 planes: U^n of the last step (RK save), or null
-    CtrlSlot* ctrl;      // device [2]
-    CtrlSlot* ctrl_host; // pinned [2]
-    double* mass_buf;    // device [MASS_BLOCKS + 1]
-    double* mass_host;   // pinned [1]
-    int parity;
-    int ty;
-    bool speed_stale;
-    bool prof;             // time each stage launch with events
-    hipEvent_t ev[3];
-    double prof_ms[2];     // accumulated stage-1 / stage-2 kernel time
-    long long prof_n;
-};
 
 namespace {
 
 bool valid(const rkg_grid* g) { return g != nullptr && g->magic == GRID_MAGIC; }
-
-double* plane_ptr(rkg_grid* g, int k) { return g->buf + (size_t)k * g->plane; }
-
-double* field_ptr(rkg_grid* g, int field) {
-    switch (field) {
-    case RKG_FIELD_H: return plane_ptr(g, 0);
-    case RKG_FIELD_HU: return plane_ptr(g, 1);
-    case RKG_FIELD_HV: return plane_ptr(g, 2);
-    case RKG_FIELD_B: return plane_ptr(g, 6);
-    case RKG_FIELD_H0: return g->save ? g->save : nullptr;
-    case RKG_FIELD_HU0: return g->save ? g->save + g->plane : nullptr;
-    case RKG_FIELD_HV0: return g->save ? g->save + 2 * g->plane : nullptr;
-    case RKG_FIELD_HS: return plane_ptr(g, 3);
-    case RKG_FIELD_HUS: return plane_ptr(g, 4);
-    case RKG_FIELD_HVS: return plane_ptr(g, 5);
-    default: return nullptr;
-    }
-}
-
-GridDesc make_desc(rkg_grid* g) {
-    GridDesc d;
-    memset(&d, 0, sizeof d);
-    const rkg_config& c = g->cfg;
-    d.nx = c.nx; d.ny = c.ny; d.pitch = g->pitch;
-    d.dx = c.dx; d.dy = c.dy;
-    d.ah = plane_ptr(g, 0); d.ahu = plane_ptr(g, 1); d.ahv = plane_ptr(g, 2);
-    d.bh = plane_ptr(g, 3); d.bhu = plane_ptr(g, 4); d.bhv = plane_ptr(g, 5);
-    d.bed = plane_ptr(g, 6);
-    d.sh = g->save;
-    d.shu = g->save ? g->save + g->plane : nullptr;
-    d.shv = g->save ? g->save + 2 * g->plane : nullptr;
-    d.g = c.g; d.cfl = c.cfl; d.manning_n = c.manning_n; d.coriolis_f = c.coriolis_f;
-    d.use_hllc = c.use_hllc;
-    for (int s = 0; s < 4; ++s) d.bc[s] = c.bc[s];
-    d.tidal_amplitude = c.tidal_amplitude; d.tidal_period = c.tidal_period;
-    d.n_tc = c.n_tidal_constituents;
-    for (int n = 0; n < RKG_MAX_TIDAL_CONSTITUENTS; ++n) {
-        d.tc_amp[n] = c.tidal_amp[n]; d.tc_phase[n] = c.tidal_phase[n]; d.tc_omega[n] = c.tidal_omega[n];
-    }
-    d.inflow_velocity = c.inflow_velocity; d.discharge_flux = c.discharge_flux;
-    d.clamped_elevation = c.clamped_elevation;
-    d.dt_fixed = c.dt_fixed; d.t_end = c.t_end;
-    d.cfl_interval = c.cfl_interval;
-    d.ty = g->ty;
-    d.nstrips = (c.nx + WAVE - 1) / WAVE;
-    d.nsegs = (c.ny + g->ty - 1) / g->ty;
-    d.ctrl = g->ctrl;
-    return d;
-}
 
 int refresh_speed(rkg_grid* g) {
     GridDesc d = make_desc(g);
@@ -153,6 +82,19 @@ int refresh_speed(rkg_grid* g) {
     return RKG_OK;
 }
 
+template <int STAGE>
+void launch_stage(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double tlim) {
+    const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
+    if (hllc && nf)
+        hipLaunchKernelGGL((k_stage<STAGE, true, true, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+    else if (hllc)
+        hipLaunchKernelGGL((k_stage<STAGE, true, false, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+    else if (nf)
+        hipLaunchKernelGGL((k_stage<STAGE, false, true, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+    else
+        hipLaunchKernelGGL((k_stage<STAGE, false, false, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+}
+
 int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
     if (g->speed_stale) {
         int rc = refresh_speed(g);
@@ -162,15 +104,9 @@ int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
     const int ntiles = d.nstrips * d.nsegs;
     for (int k = 0; k < n; ++k) {
         if (g->prof) HIP_TRY(hipEventRecord(g->ev[0], g->stream));
-        if (g->cfg.use_hllc)
-            hipLaunchKernelGGL((k_stage<1, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-        else
-            hipLaunchKernelGGL((k_stage<1, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        launch_stage<1>(g, d, ntiles, mode, tlim);
         if (g->prof) HIP_TRY(hipEventRecord(g->ev[1], g->stream));
-        if (g->cfg.use_hllc)
-            hipLaunchKernelGGL((k_stage<2, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-        else
-            hipLaunchKernelGGL((k_stage<2, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        launch_stage<2>(g, d, ntiles, mode, tlim);
         g->parity ^= 1;
         if (g->prof) {
             float a = 0.f, b = 0.f;
@@ -286,6 +222,7 @@ void rkg_destroy(rkg_grid** pg) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->buf) (void)hipFree(g->buf);
     if (g->save) (void)hipFree(g->save);
+    if (g->nfield) (void)hipFree(g->nfield);
     if (g->ctrl) (void)hipFree(g->ctrl);
     if (g->mass_buf) (void)hipFree(g->mass_buf);
     if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
@@ -302,7 +239,7 @@ static int copy_field(rkg_grid* g, int field, double* host_or_dev, int ld, bool 
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     double* f = field_ptr(g, field);
     if (!f) { set_error("unknown or unallocated field %d", field); return RKG_ERR_RANGE; }
-    if (upload && field > RKG_FIELD_B) { set_error("field %d is read-only", field); return RKG_ERR_RANGE; }
+    if (upload && field > RKG_FIELD_B && field != RKG_FIELD_N) { set_error("field %d is read-only", field); return RKG_ERR_RANGE; }
     if (!host_or_dev || ld < g->cfg.nx) { set_error("bad buffer or leading dimension"); return RKG_ERR_RANGE; }
     const size_t w = (size_t)g->cfg.nx * sizeof(double);
     if (upload) {
@@ -429,6 +366,27 @@ int rkg_set_rk_save(rkg_grid* g, int enable) {
         HIP_TRY(hipStreamSynchronize(g->stream));
         HIP_TRY(hipFree(g->save));
         g->save = nullptr;
+    }
+    return RKG_OK;
+}
+
+// Fills a plane with one value.
+__global__ void k_fill(double* p, long long n, double v) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
+}
+
+int rkg_use_manning_field(rkg_grid* g, int enable) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (enable && !g->nfield) {
+        HIP_TRY(hipMalloc((void**)&g->nfield, g->plane * sizeof(double)));
+        hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, g->stream, g->nfield, (long long)g->plane,
+                           g->cfg.manning_n);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    } else if (!enable && g->nfield) {
+        HIP_TRY(hipFree(g->nfield));
+        g->nfield = nullptr;
     }
     return RKG_OK;
 }

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(PERSISTENT
     if (!PERSISTENT) {
         const int t = xcd_tile(blockIdx.x, ntiles);
         const BatchTile bt = tiles[t];
-        stage_tile<STAGE, HLLC>(descs[bt.member], L, p, MODE_STEP, descs[bt.member].t_end, [&](const GridDesc&) {
+        stage_tile<STAGE, HLLC, false, false>(descs[bt.member], L, p, MODE_STEP, descs[bt.member].t_end, [&](const GridDesc&) {
             return make_int3(bt.strip, bt.seg, bt.leader);
         });
         return;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(PERSISTENT
         const int t = __builtin_amdgcn_readfirstlane((int)slot);
         if (t >= ntiles) break;
         const BatchTile bt = tiles[t];
-        stage_tile<STAGE, HLLC>(descs[bt.member], L, p, MODE_STEP, descs[bt.member].t_end, [&](const GridDesc&) {
+        stage_tile<STAGE, HLLC, false, false>(descs[bt.member], L, p, MODE_STEP, descs[bt.member].t_end, [&](const GridDesc&) {
             return make_int3(bt.strip, bt.seg, bt.leader);
         });
         __syncthreads();   // the next tile rewrites the LDS buffers and the slot
@@ -115,6 +115,8 @@ GridDesc ens_desc(rkg_ensemble* e, int m) {
     d.ah = ens_plane(e, m, 0); d.ahu = ens_plane(e, m, 1); d.ahv = ens_plane(e, m, 2);
     d.bh = ens_plane(e, m, 3); d.bhu = ens_plane(e, m, 4); d.bhv = ens_plane(e, m, 5);
     d.bed = ens_plane(e, m, 6);
+    d.o2h = d.ah; d.o2hu = d.ahu; d.o2hv = d.ahv;
+    d.plane = (long long)e->plane[m];
     d.g = c.g; d.cfl = c.cfl; d.manning_n = c.manning_n; d.coriolis_f = c.coriolis_f;
     d.use_hllc = c.use_hllc;
     for (int s = 0; s < 4; ++s) d.bc[s] = c.bc[s];

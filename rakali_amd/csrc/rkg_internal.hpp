@@ -4,10 +4,99 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
+#include "../../include/rakali_amd.h"
+#include "swe_kernels.hpp"
+
 namespace rk {
 
 // Records the message returned by rkg_last_error() on this thread.
 __attribute__((visibility("hidden"), format(printf, 1, 2))) void set_error(const char* fmt, ...);
+
+}  // namespace rk
+
+// One grid: device buffers, stream and control slots (rkg_grid of the ABI).
+struct rkg_grid {
+    unsigned magic;
+    rkg_config cfg;
+    int device;
+    hipStream_t stream;
+    bool own_stream;
+    long long pitch;
+    size_t plane;        // elements per field plane
+    double* buf;         // 7 planes: A(h,hu,hv) B(h,hu,hv) bed
+    double* save;        // 3 planes: U^n of the last step (RK save), or null
+    double* nfield;      // per-cell Manning n (1 plane), or null
+    rk::CtrlSlot* ctrl;      // device [2]
+    rk::CtrlSlot* ctrl_host; // pinned [2]
+    double* mass_buf;    // device [MASS_BLOCKS + 1]
+    double* mass_host;   // pinned [1]
+    int parity;
+    int ty;
+    bool speed_stale;
+    bool prof;             // time each stage launch with events
+    hipEvent_t ev[3];
+    double prof_ms[2];     // accumulated stage-1 / stage-2 kernel time
+    long long prof_n;
+};
+
+namespace rk {
+
+inline double* plane_ptr(rkg_grid* g, int k) { return g->buf + (size_t)k * g->plane; }
+
+inline double* field_ptr(rkg_grid* g, int field) {
+    switch (field) {
+    case RKG_FIELD_H: return plane_ptr(g, 0);
+    case RKG_FIELD_HU: return plane_ptr(g, 1);
+    case RKG_FIELD_HV: return plane_ptr(g, 2);
+    case RKG_FIELD_B: return plane_ptr(g, 6);
+    case RKG_FIELD_H0: return g->save ? g->save : nullptr;
+    case RKG_FIELD_HU0: return g->save ? g->save + g->plane : nullptr;
+    case RKG_FIELD_HV0: return g->save ? g->save + 2 * g->plane : nullptr;
+    case RKG_FIELD_HS: return plane_ptr(g, 3);
+    case RKG_FIELD_HUS: return plane_ptr(g, 4);
+    case RKG_FIELD_HVS: return plane_ptr(g, 5);
+    case RKG_FIELD_N: return g->nfield;
+    default: return nullptr;
+    }
+}
+
+inline GridDesc make_desc(rkg_grid* g) {
+    GridDesc d;
+    memset(&d, 0, sizeof d);
+    const rkg_config& c = g->cfg;
+    d.nx = c.nx; d.ny = c.ny; d.pitch = g->pitch;
+    d.dx = c.dx; d.dy = c.dy;
+    d.ah = plane_ptr(g, 0); d.ahu = plane_ptr(g, 1); d.ahv = plane_ptr(g, 2);
+    d.bh = plane_ptr(g, 3); d.bhu = plane_ptr(g, 4); d.bhv = plane_ptr(g, 5);
+    d.bed = plane_ptr(g, 6);
+    d.o2h = d.ah; d.o2hu = d.ahu; d.o2hv = d.ahv;
+    d.nfield = g->nfield;
+    d.rec = nullptr;
+    d.plane = (long long)g->plane;
+    d.sh = g->save;
+    d.shu = g->save ? g->save + g->plane : nullptr;
+    d.shv = g->save ? g->save + 2 * g->plane : nullptr;
+    d.g = c.g; d.cfl = c.cfl; d.manning_n = c.manning_n; d.coriolis_f = c.coriolis_f;
+    d.use_hllc = c.use_hllc;
+    for (int s = 0; s < 4; ++s) d.bc[s] = c.bc[s];
+    d.tidal_amplitude = c.tidal_amplitude; d.tidal_period = c.tidal_period;
+    d.n_tc = c.n_tidal_constituents;
+    for (int n = 0; n < RKG_MAX_TIDAL_CONSTITUENTS; ++n) {
+        d.tc_amp[n] = c.tidal_amp[n]; d.tc_phase[n] = c.tidal_phase[n]; d.tc_omega[n] = c.tidal_omega[n];
+    }
+    d.inflow_velocity = c.inflow_velocity; d.discharge_flux = c.discharge_flux;
+    d.clamped_elevation = c.clamped_elevation;
+    d.dt_fixed = c.dt_fixed; d.t_end = c.t_end;
+    d.cfl_interval = c.cfl_interval;
+    d.ty = g->ty;
+    d.nstrips = (c.nx + WAVE - 1) / WAVE;
+    d.nsegs = (c.ny + g->ty - 1) / g->ty;
+    d.ctrl = g->ctrl;
+    return d;
+}
+
 
 }  // namespace rk
 

@@ -59,7 +59,7 @@ __device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, i
 // One tile of one stage of one grid.  `where(d)` gives the tile as
 // (strip, segment, leader); the leader is the single tile per grid and
 // launch that advances the control slot.
-template <int STAGE, bool HLLC, class Where>
+template <int STAGE, bool HLLC, bool NF, bool REC, class Where>
 __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p, int mode, double tlim,
                                            Where where) {
     const int3 w = where(d);
@@ -87,7 +87,7 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
         const CtrlSlot cq = d.ctrl[1 - p];
         if (!cq.active) return;
         s.ih = d.bh; s.ihu = d.bhu; s.ihv = d.bhv;
-        s.oh = d.ah; s.ohu = d.ahu; s.ohv = d.ahv;
+        s.oh = d.o2h; s.ohu = d.o2hu; s.ohv = d.o2hv;
         s.dt = cq.dt;
         s.eta_tidal = tidal_eta(d, cq.t);  // bc%time = t + dt
     }
@@ -97,9 +97,9 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     const int strip = w.x, seg = w.y;
     double smax;
     if (tile_is_interior(d, strip, seg))
-        smax = process_tile<STAGE, HLLC, false>(d, s, L, strip, seg);
+        smax = process_tile<STAGE, HLLC, false, NF, REC>(d, s, L, strip, seg);
     else
-        smax = process_tile<STAGE, HLLC, true>(d, s, L, strip, seg);
+        smax = process_tile<STAGE, HLLC, true, NF, REC>(d, s, L, strip, seg);
     if (STAGE == 2) {
         smax = wave_max(smax);
         if (threadIdx.x == 0 && smax > 0.0)
@@ -108,10 +108,10 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
 }
 
 // Single grid: one workgroup (one wavefront) per tile.
-template <int STAGE, bool HLLC>
+template <int STAGE, bool HLLC, bool NF, bool REC>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage(GridDesc d, int p, int mode, double tlim) {
     __shared__ TileLds L;
-    stage_tile<STAGE, HLLC>(d, L, p, mode, tlim, [](const GridDesc& g) {
+    stage_tile<STAGE, HLLC, NF, REC>(d, L, p, mode, tlim, [](const GridDesc& g) {
         const int tile = xcd_tile(blockIdx.x, g.nstrips * g.nsegs);
         return make_int3(tile % g.nstrips, tile / g.nstrips, blockIdx.x == 0);
     });

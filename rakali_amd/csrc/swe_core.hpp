@@ -302,21 +302,24 @@ struct SourceTerms {
 };
 
 // Euler update + clamp + Manning + Coriolis, rki_kernel_fused.F90:228-290
-// (forcing inactive).
-template <bool EXACT>
+// (forcing inactive).  With NF the Manning coefficient is the cell's own
+// n_cell (friction where n_cell > 0) instead of the scalar one.
+template <bool EXACT, bool NF>
 __device__ __forceinline__ void stage_body(const SourceTerms& s, double dt, double fh, double fhu,
-                                           double fhv, double& h, double& hu, double& hv) {
+                                           double fhv, double& h, double& hu, double& hv, double n_cell) {
     h = h + dt * fh;
     hu = hu + dt * fhu;
     hv = hv + dt * fhv;
     clamp_desing<EXACT>(s.g, h, hu, hv);
+    const bool friction = NF ? (n_cell > 0.0) : (s.has_friction != 0);
+    const double n_sq = NF ? n_cell * n_cell : s.n_sq;
     if (h > DRY_TOLERANCE) {
-        if (s.has_friction) {
+        if (friction) {
             const Recip rh = recip(h);
             const double u = rdiv<EXACT>(hu, rh);
             const double v = rdiv<EXACT>(hv, rh);
             const double speed = sqrt(u * u + v * v);
-            const double cf = s.g * s.n_sq / crm::pow13(h);
+            const double cf = s.g * n_sq / crm::pow13(h);
             const Recip rd = recip(1.0 + rdiv<EXACT>(dt * cf * speed, rh));
             hu = rdiv<EXACT>(hu, rd);
             hv = rdiv<EXACT>(hv, rd);

@@ -36,7 +36,11 @@ struct GridDesc {
     double* ah; double* ahu; double* ahv;   // A: U^n / U^{n+1}
     double* bh; double* bhu; double* bhv;   // B: U*
     double* sh; double* shu; double* shv;   // optional copy of U^n (RK save), or null
+    double* o2h; double* o2hu; double* o2hv; // stage-2 output (normally A itself)
     const double* bed;
+    const double* nfield;         // per-cell Manning n, or null (scalar manning_n)
+    double* rec;                  // 3 planes receiving each cell's flux divergence, or null
+    long long plane;              // elements per plane (pitch * ny)
     double g, cfl, manning_n, coriolis_f;
     int use_hllc;
     int bc[4];                    // west, east, south, north
@@ -243,7 +247,7 @@ struct ExactTag {
 // shared-reciprocal ones; both give the IEEE quotients.
 // STAGE 1 writes U* to B; STAGE 2 reads U* and U^n, applies the RK2 average
 // and writes U^{n+1} over A, returning the lane's max CFL speed.
-template <int STAGE, bool HLLC, bool EDGE>
+template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC>
 __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx& s, TileLds& L,
                                                int strip, int seg) {
     const int lane = threadIdx.x;
@@ -352,8 +356,15 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
             if (own) {
                 double fh, fhu, fhv;
                 divergence<EX>(g, s.rdx, s.rdy, fxl, fxr, fyb, fyt, fh, fhu, fhv);
+                const long long k = (long long)j * pitch + c;
+                if (REC) {
+                    d.rec[k] = fh; d.rec[d.plane + k] = fhu; d.rec[2 * d.plane + k] = fhv;
+                }
                 double h = rc.h, hu = rc.hu, hv = rc.hv;
-                stage_body<EX>(s.src, s.dt, fh, fhu, fhv, h, hu, hv);
+                if (NF)
+                    stage_body<EX, true>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, d.nfield[k]);
+                else
+                    stage_body<EX, false>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, 0.0);
                 if (STAGE == 2) {
                     // rki_kernel_fused.F90:435-463
                     h = 0.5 * (un.h + h);
@@ -362,7 +373,6 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                     clamp_desing<EX>(g, h, hu, hv);
                     smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
                 }
-                const long long k = (long long)j * pitch + c;
                 s.oh[k] = h; s.ohu[k] = hu; s.ohv[k] = hv;
                 if (STAGE == 1 && d.sh != nullptr) {  // h0, hu0, hv0 (rki_kernel_fused.F90:224-226)
                     d.sh[k] = rc.h; d.shu[k] = rc.hu; d.shv[k] = rc.hv;

@@ -107,6 +107,10 @@ class Grid:
                                                      out.ctypes.data_as(ctypes.c_void_p), self.nx))
         return out
 
+    def use_manning_field(self, enable: bool = True) -> None:
+        """Per-cell Manning n (field "n", initialised to the scalar value)."""
+        check("rkg_use_manning_field", self._lib.rkg_use_manning_field(self.handle, int(bool(enable))))
+
     def device_ptr(self, name: str):
         p = ctypes.c_void_p()
         pitch = ctypes.c_longlong()
