@@ -73,6 +73,12 @@ extern "C" {
 
 #define RKG_MAX_TIDAL_CONSTITUENTS 10
 
+/* Explicit time schemes, tags as TIME_* in rki_timestepper.F90:27-33. */
+#define RKG_TIME_EULER 1
+#define RKG_TIME_RK2 2
+#define RKG_TIME_RK3 3
+#define RKG_TIME_RK43 4
+
 /* Scalar configuration of one grid; the namelist keys the step reads
  * (config_t, rki_config.F90:37-260; bc_params_t, rki_boundary_types.F90:37). */
 typedef struct rkg_config {
@@ -150,6 +156,22 @@ int rkg_set_rk_save(rkg_grid *g, int enable);
  * initialised to the scalar manning_n; friction where n > 0); 0: back to
  * the scalar.  No counterpart in the reference (scalar n only). */
 int rkg_use_manning_field(rkg_grid *g, int enable);
+/* Time scheme of rkg_step / rkg_advance (state%time_scheme, set from the
+ * namelist key time_scheme, rki_config.F90).  RKG_TIME_RK2 (default) is
+ * the fused SSP-RK2 path (rki_solver.F90:760-815); euler, rk3 and rk43 run
+ * the generic SSP stage loop (solver_advance_2d_generic,
+ * rki_solver.F90:861-925, coefficients rki_timestepper.F90:60-100; rk43
+ * doubles the CFL dt).  rk3 and rk43 allocate three more planes. */
+int rkg_set_time_scheme(rkg_grid *g, int scheme);
+int rkg_get_time_scheme(rkg_grid *g, int *scheme);
+/* Boundary mass budget state%barotropic%mass_out
+ * (accumulate_boundary_mass_out, rki_solver.F90:507-537): while enabled,
+ * every SSP-RK2 step subtracts 0.5*dt*sum(flux_h)*dx*dy per stage, the
+ * volume that left through the boundary.  Enabling starts it at 0; the
+ * generic schemes do not accumulate it (as in the reference).  The sum is
+ * in a fixed order (per tile, then over tiles). */
+int rkg_set_mass_budget(rkg_grid *g, int enable);
+int rkg_get_mass_budget(rkg_grid *g, double *mass_out);
 /* Reads / replaces the scalar configuration (nx and ny cannot change):
  * boundary values, tide, friction, CFL, t_end, HLL/HLLC. */
 int rkg_get_config(rkg_grid *g, rkg_config *cfg);

@@ -28,6 +28,9 @@ BC_NAMES = {"wall": BC_WALL, "open": BC_OPEN, "tidal": BC_TIDAL,
             "sponge": BC_SPONGE, "chapman": BC_CHAPMAN}
 
 MAX_TC = 10
+# time schemes (rki_timestepper.F90:27-33); RK2_GENERIC runs Heun's
+# coefficients through the generic stage loop
+TIME_EULER, TIME_RK2, TIME_RK3, TIME_RK43, TIME_RK2_GENERIC = 1, 2, 3, 4, 102
 
 
 class Params(ctypes.Structure):
@@ -47,6 +50,7 @@ class Params(ctypes.Structure):
         ("inflow_velocity", ctypes.c_double),
         ("discharge_flux", ctypes.c_double),
         ("clamped_elevation", ctypes.c_double),
+        ("time_scheme", ctypes.c_int),
     ]
 
 

@@ -87,6 +87,7 @@ void orc_default_params(orc_params *p)
     p->bc[0] = p->bc[1] = p->bc[2] = p->bc[3] = ORC_BC_WALL;
     p->tidal_amplitude = 0.0;
     p->tidal_period = 44712.0;           /* rki_boundary_types.F90 */
+    p->time_scheme = ORC_TIME_RK2;       /* rki_timestepper.F90:47 */
 }
 
 /* ---------------------------------------------------------------------- */
@@ -505,6 +506,49 @@ void orc_fused_stage2(const orc_params *p, double *h, double *hu, double *hv,
     }
 }
 
+/* fused_update_stage_generic, rki_kernel_fused.F90:22-162: the stage-1
+ * body without the U^n save, at the stage's dt (beta_i * dt). */
+void orc_fused_stage_generic(const orc_params *p, double *h, double *hu, double *hv,
+                             const double *fh, const double *fhu, const double *fhv,
+                             double dt)
+{
+    STAGE_SETUP
+    int j;
+#pragma omp parallel for schedule(static) num_threads(orc_get_threads())
+    for (j = ng + 1; j <= nyt - ng; ++j) {
+        for (int i = ng + 1; i <= nxt - ng; ++i) {
+            size_t k = IX(i, j);
+            double hh = h[k], hhu = hu[k], hhv = hv[k];
+            stage_body(p, dt, eps4, n_sq, has_friction, has_coriolis, cosf, sinf,
+                       fh[k], fhu[k], fhv[k], &hh, &hhu, &hhv);
+            h[k] = hh; hu[k] = hhu; hv[k] = hhv;
+        }
+    }
+}
+
+/* compute_ssp_combine, rki_kernel_update.F90:152-208:
+ * U = a_n U^n + (1 - a_n) U, then the clamp / KP07 / backstop. */
+void orc_ssp_combine(const orc_params *p, double *h, double *hu, double *hv,
+                     const double *h0, const double *hu0, const double *hv0,
+                     double a_n)
+{
+    const int nxt = NT(p), nyt = MT(p), ng = p->ng;
+    const double eps4 = pow4(DESING_EPS);
+    const double a_prev = 1.0 - a_n;
+    int j;
+#pragma omp parallel for schedule(static) num_threads(orc_get_threads())
+    for (j = ng + 1; j <= nyt - ng; ++j) {
+        for (int i = ng + 1; i <= nxt - ng; ++i) {
+            size_t k = IX(i, j);
+            double hh = a_n * h0[k] + a_prev * h[k];
+            double hhu = a_n * hu0[k] + a_prev * hu[k];
+            double hhv = a_n * hv0[k] + a_prev * hv[k];
+            clamp_desing(p->g, eps4, &hh, &hhu, &hhv);
+            h[k] = hh; hu[k] = hhu; hv[k] = hhv;
+        }
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* CFL timestep (rki_kernel_timestep.F90:16-78)                           */
 /* ---------------------------------------------------------------------- */
@@ -741,6 +785,27 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
     double *fh = work, *fhu = work + n, *fhv = work + 2 * n;
     double *h0 = work + 3 * n, *hu0 = work + 4 * n, *hv0 = work + 5 * n;
     const double area = p->dx * p->dy;
+    /* Shu-Osher table, make_timestepper (rki_timestepper.F90:60-100) */
+    const int generic = p->time_scheme != ORC_TIME_RK2;
+    int n_stages = 2;
+    double ssp_coeff = 1.0, alpha[4] = {0.0, 0.5, 0.0, 0.0}, beta[4] = {1.0, 1.0, 0.0, 0.0},
+           cabs[4] = {0.0, 1.0, 0.0, 0.0};
+    if (p->time_scheme == ORC_TIME_EULER) {
+        n_stages = 1; alpha[0] = 0.0; beta[0] = 1.0; cabs[0] = 0.0;
+    } else if (p->time_scheme == ORC_TIME_RK3) {
+        n_stages = 3;
+        alpha[0] = 0.0; alpha[1] = 0.75; alpha[2] = 1.0 / 3.0;
+        beta[0] = beta[1] = beta[2] = 1.0;
+        cabs[0] = 0.0; cabs[1] = 1.0; cabs[2] = 0.5;
+    } else if (p->time_scheme == ORC_TIME_RK43) {
+        n_stages = 4; ssp_coeff = 2.0;
+        alpha[0] = 0.0; alpha[1] = 0.0; alpha[2] = 2.0 / 3.0; alpha[3] = 0.0;
+        beta[0] = beta[1] = beta[2] = beta[3] = 0.5;
+        cabs[0] = 0.0; cabs[1] = 0.5; cabs[2] = 1.0; cabs[3] = 0.5;
+    } else if (p->time_scheme != ORC_TIME_RK2 && p->time_scheme != ORC_TIME_RK2_GENERIC) {
+        free(work);
+        return -2;
+    }
     long long taken = 0;
     for (;;) {
         double dt_max;
@@ -758,11 +823,27 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
             clk->dt = dmin(p->dt_fixed, dt_max);
         } else if (clk->step_count % p->cfl_interval == 0 || clk->dt <= 0.0) {
             clk->dt = orc_compute_timestep(p, h, hu, hv);
-            clk->dt = clk->dt * 1.0; /* ssp_coeff of RK2 */
+            clk->dt = clk->dt * ssp_coeff; /* :736-741 */
             clk->dt = dmin(clk->dt, dt_max);
         }
         clk->step_count += 1;
         const double dt = clk->dt;
+        if (generic) {
+            /* solver_advance_2d_generic, rki_solver.F90:861-925 */
+            memcpy(h0, h, n * sizeof(double));
+            memcpy(hu0, hu, n * sizeof(double));
+            memcpy(hv0, hv, n * sizeof(double));
+            for (int st = 0; st < n_stages; ++st) {
+                const double dt_stage = beta[st] * dt;
+                orc_apply_bc(p, time + cabs[st] * dt, h, hu, hv, b);
+                orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);
+                orc_fused_stage_generic(p, h, hu, hv, fh, fhu, fhv, dt_stage);
+                if (alpha[st] != 0.0) orc_ssp_combine(p, h, hu, hv, h0, hu0, hv0, alpha[st]);
+            }
+            clk->t = clk->t + dt;
+            ++taken;
+            continue;
+        }
         /* stage 1 :763-787 */
         orc_apply_bc(p, time, h, hu, hv, b);
         orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);

@@ -39,7 +39,14 @@ typedef struct {
     int n_tc;
     double tc_amp[ORC_MAX_TC], tc_phase[ORC_MAX_TC], tc_omega[ORC_MAX_TC];
     double inflow_velocity, discharge_flux, clamped_elevation;
+    int time_scheme;         /* ORC_TIME_* (rki_timestepper.F90:27-36) */
 } orc_params;
+
+/* Explicit SSP schemes, tags as rki_timestepper.F90:27-33.  ORC_TIME_RK2
+ * runs the fused RK2 sequence; ORC_TIME_RK2_GENERIC runs the same Heun
+ * coefficients through the generic stage loop (a check of that loop). */
+enum { ORC_TIME_EULER = 1, ORC_TIME_RK2 = 2, ORC_TIME_RK3 = 3, ORC_TIME_RK43 = 4,
+       ORC_TIME_RK2_GENERIC = 102 };
 
 typedef struct {
     double t;            /* handle t_current */
@@ -63,6 +70,14 @@ void orc_fused_stage2(const orc_params *p, double *h, double *hu, double *hv,
                       const double *h0, const double *hu0, const double *hv0,
                       const double *fh, const double *fhu, const double *fhv,
                       double dt);
+/* fused_update_stage_generic (rki_kernel_fused.F90:22-162) and
+ * compute_ssp_combine (rki_kernel_update.F90:152-208). */
+void orc_fused_stage_generic(const orc_params *p, double *h, double *hu, double *hv,
+                             const double *fh, const double *fhu, const double *fhv,
+                             double dt);
+void orc_ssp_combine(const orc_params *p, double *h, double *hu, double *hv,
+                     const double *h0, const double *hu0, const double *hv0,
+                     double a_n);
 double orc_compute_timestep(const orc_params *p, const double *h,
                             const double *hu, const double *hv);
 double orc_tidal_eta(const orc_params *p, double time);
@@ -73,7 +88,8 @@ void orc_init_still_water(const orc_params *p, double eta0, const double *b,
                           double *h, double *hu, double *hv);
 double orc_total_mass(const orc_params *p, const double *h);
 
-/* The fused SSP-RK2 step sequence of rki_solver.F90:692-815 driven by
+/* The step sequence of rki_solver.F90:692-815 (fused SSP-RK2) or, for the
+ * other time schemes, :861-925 (generic SSP stage loop), driven by
  * rki_step (mode 0, n steps, dt_max = t_end - t; rki_api.F90:358-383) or
  * rki_advance (mode 1, loop while t < target - 1e-10; rki_api.F90:385-410).
  * Returns the number of steps taken, or a negative error. */

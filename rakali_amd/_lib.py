@@ -15,6 +15,8 @@ _PKG = Path(__file__).resolve().parent
 # reference's librakali, python/rakali/_core.py:28-44).
 LIB_PATH = Path(os.environ.get("RAKALI_AMD_LIB") or (_PKG / "librakali_amd.so"))
 
+# time schemes, rki_timestepper.F90:27-33
+TIME_SCHEMES = {"euler": 1, "rk2": 2, "rk3": 3, "rk43": 4}
 MAX_TC = 10
 
 RKG_OK = 0
@@ -87,6 +89,10 @@ SIGNATURES = {
     "rkg_copy_bandwidth": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, _dp, _dp]),
     "rkg_set_rk_save": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_use_manning_field": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_set_time_scheme": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_get_time_scheme": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "rkg_set_mass_budget": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_get_mass_budget": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
     "rkg_get_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_set_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_ensemble_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, _P, ctypes.POINTER(_P)]),

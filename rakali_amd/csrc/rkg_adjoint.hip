@@ -444,6 +444,7 @@ GridDesc stage_desc(rkg_adjoint* a, double* in, double* in2, double* out2) {
     d.bh = in2; d.bhu = in2 + P(a); d.bhv = in2 + 2 * P(a);
     d.o2h = out2; d.o2hu = out2 + P(a); d.o2hv = out2 + 2 * P(a);
     d.sh = d.shu = d.shv = nullptr;
+    d.mpart = nullptr;
     return d;
 }
 

@@ -73,7 +73,7 @@ int refresh_speed(rkg_grid* g) {
     HIP_TRY(hipStreamSynchronize(g->stream));
     const CtrlSlot c = g->ctrl_host[g->parity];
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, c.t, c.dt,
-                       c.step_count);
+                       c.step_count, c.mass_out);
     const long long n = (long long)g->cfg.nx * g->cfg.ny;
     int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_init_speed, dim3(blocks), dim3(256), 0, g->stream, d, g->parity);
@@ -95,18 +95,96 @@ void launch_stage(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double t
         hipLaunchKernelGGL((k_stage<STAGE, false, false, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
 }
 
+// Shu-Osher table of a generic scheme, make_timestepper
+// (rki_timestepper.F90:60-100); ssp_coeff is in make_desc.
+struct SchemeTab {
+    int n;
+    double alpha[4], beta[4], c[4];
+};
+
+SchemeTab scheme_tab(int scheme) {
+    SchemeTab t{};
+    if (scheme == RKG_TIME_EULER) {
+        t.n = 1; t.alpha[0] = 0.0; t.beta[0] = 1.0; t.c[0] = 0.0;
+    } else if (scheme == RKG_TIME_RK3) {
+        t.n = 3;
+        t.alpha[0] = 0.0; t.alpha[1] = 0.75; t.alpha[2] = 1.0 / 3.0;
+        t.beta[0] = 1.0; t.beta[1] = 1.0; t.beta[2] = 1.0;
+        t.c[0] = 0.0; t.c[1] = 1.0; t.c[2] = 0.5;
+    } else {  // RKG_TIME_RK43
+        t.n = 4;
+        t.alpha[0] = 0.0; t.alpha[1] = 0.0; t.alpha[2] = 2.0 / 3.0; t.alpha[3] = 0.0;
+        t.beta[0] = 0.5; t.beta[1] = 0.5; t.beta[2] = 0.5; t.beta[3] = 0.5;
+        t.c[0] = 0.0; t.c[1] = 0.5; t.c[2] = 1.0; t.c[3] = 0.5;
+    }
+    return t;
+}
+
+void launch_gen(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, int mode, double tlim) {
+    const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
+    if (hllc && nf)
+        hipLaunchKernelGGL((k_stage_gen<true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+    else if (hllc)
+        hipLaunchKernelGGL((k_stage_gen<true, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+    else if (nf)
+        hipLaunchKernelGGL((k_stage_gen<false, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+    else
+        hipLaunchKernelGGL((k_stage_gen<false, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+}
+
+// Generic explicit SSP steps (euler / rk3 / rk43), solver_advance_2d_generic
+// (rki_solver.F90:861-925).  Stage i reads the previous iterate and writes
+// B or C in turn; the last stage writes A, which holds U^n until then.  A
+// one-stage scheme writes B and k_euler_finish copies it to A.
+int launch_generic(rkg_grid* g, int n, int mode, double tlim) {
+    GridDesc d = make_desc(g);
+    d.mpart = nullptr;  // the generic path keeps no mass budget (as the reference)
+    const int ntiles = d.nstrips * d.nsegs;
+    const SchemeTab T = scheme_tab(g->scheme);
+    double* A = plane_ptr(g, 0);
+    double* B = plane_ptr(g, 3);
+    double* C = g->cbuf;
+    const size_t P = g->plane;
+    for (int k = 0; k < n; ++k) {
+        const double* in = A;
+        for (int st = 0; st < T.n; ++st) {
+            double* out = (T.n > 1 && st == T.n - 1) ? A : ((st % 2 == 0) ? B : C);
+            GenStage gs;
+            gs.ih = in; gs.ihu = in + P; gs.ihv = in + 2 * P;
+            gs.oh = out; gs.ohu = out + P; gs.ohv = out + 2 * P;
+            gs.alpha = T.alpha[st]; gs.beta = T.beta[st]; gs.c = T.c[st];
+            gs.first = st == 0;
+            gs.last = T.n > 1 && st == T.n - 1;
+            launch_gen(g, d, gs, ntiles, mode, tlim);
+            in = out;
+        }
+        if (T.n == 1) {
+            const long long cells = (long long)d.nx * d.ny;
+            const int blocks = (int)std::min<long long>((cells + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_euler_finish, dim3(blocks), dim3(256), 0, g->stream, d, g->parity);
+        }
+        g->parity ^= 1;
+    }
+    HIP_TRY(hipGetLastError());
+    return RKG_OK;
+}
+
 int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
     if (g->speed_stale) {
         int rc = refresh_speed(g);
         if (rc) return rc;
     }
+    if (g->scheme != RKG_TIME_RK2) return launch_generic(g, n, mode, tlim);
     GridDesc d = make_desc(g);
     const int ntiles = d.nstrips * d.nsegs;
+    const double area = d.dx * d.dy;
     for (int k = 0; k < n; ++k) {
         if (g->prof) HIP_TRY(hipEventRecord(g->ev[0], g->stream));
         launch_stage<1>(g, d, ntiles, mode, tlim);
         if (g->prof) HIP_TRY(hipEventRecord(g->ev[1], g->stream));
         launch_stage<2>(g, d, ntiles, mode, tlim);
+        if (d.mpart)
+            hipLaunchKernelGGL(k_budget, dim3(1), dim3(256), 0, g->stream, g->ctrl, g->parity, d.mpart, ntiles, area);
         g->parity ^= 1;
         if (g->prof) {
             float a = 0.f, b = 0.f;
@@ -180,6 +258,7 @@ int rkg_create(const rkg_config* cfg, void* stream, rkg_grid** out) {
     memset(g, 0, sizeof *g);
     g->cfg = *cfg;
     g->ty = 32;
+    g->scheme = RKG_TIME_RK2;
     HIP_TRY(hipGetDevice(&g->device));
     if (stream) {
         g->stream = (hipStream_t)stream;
@@ -223,6 +302,8 @@ void rkg_destroy(rkg_grid** pg) {
     if (g->buf) (void)hipFree(g->buf);
     if (g->save) (void)hipFree(g->save);
     if (g->nfield) (void)hipFree(g->nfield);
+    if (g->cbuf) (void)hipFree(g->cbuf);
+    if (g->mpart) (void)hipFree(g->mpart);
     if (g->ctrl) (void)hipFree(g->ctrl);
     if (g->mass_buf) (void)hipFree(g->mass_buf);
     if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
@@ -322,7 +403,8 @@ int rkg_set_clock(rkg_grid* g, double t, long long step_count) {
     CtrlSlot c;
     int rc = read_clock(g, &c);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, t, c.dt, step_count);
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, t, c.dt, step_count,
+                       c.mass_out);
     HIP_TRY(hipGetLastError());
     g->speed_stale = true;
     return RKG_OK;
@@ -353,6 +435,61 @@ int rkg_set_tile_rows(rkg_grid* g, int rows) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     if (rows < 1 || rows > WAVE) { set_error("tile rows must be in 1..64"); return RKG_ERR_RANGE; }
     g->ty = rows;
+    return RKG_OK;
+}
+
+int rkg_set_time_scheme(rkg_grid* g, int scheme) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (scheme < RKG_TIME_EULER || scheme > RKG_TIME_RK43) {
+        set_error("time scheme must be 1 (euler), 2 (rk2), 3 (rk3) or 4 (rk43); got %d", scheme);
+        return RKG_ERR_RANGE;
+    }
+    if ((scheme == RKG_TIME_RK3 || scheme == RKG_TIME_RK43) && !g->cbuf) {
+        HIP_TRY(hipMalloc((void**)&g->cbuf, 3 * g->plane * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(g->cbuf, 0, 3 * g->plane * sizeof(double), g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    g->scheme = scheme;
+    return RKG_OK;
+}
+
+int rkg_get_time_scheme(rkg_grid* g, int* scheme) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!scheme) return RKG_ERR_RANGE;
+    *scheme = g->scheme;
+    return RKG_OK;
+}
+
+int rkg_set_mass_budget(rkg_grid* g, int enable) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (enable && !g->mpart) {
+        // room for the tiles of both stages at any tile height
+        const long long cap = (long long)((g->cfg.nx + WAVE - 1) / WAVE) * g->cfg.ny;
+        HIP_TRY(hipMalloc((void**)&g->mpart, 2 * cap * sizeof(double)));
+        g->mpart_cap = cap;
+        CtrlSlot c;
+        int rc = read_clock(g, &c);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, c.t, c.dt,
+                           c.step_count, 0.0);
+        HIP_TRY(hipGetLastError());
+        g->speed_stale = true;
+    } else if (!enable && g->mpart) {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipFree(g->mpart));
+        g->mpart = nullptr;
+        g->mpart_cap = 0;
+    }
+    return RKG_OK;
+}
+
+int rkg_get_mass_budget(rkg_grid* g, double* mass_out) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!mass_out) return RKG_ERR_RANGE;
+    CtrlSlot c;
+    int rc = read_clock(g, &c);
+    if (rc) return rc;
+    *mass_out = c.mass_out;
     return RKG_OK;
 }
 

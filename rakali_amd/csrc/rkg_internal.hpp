@@ -28,6 +28,10 @@ struct rkg_grid {
     double* buf;         // 7 planes: A(h,hu,hv) B(h,hu,hv) bed
     double* save;        // 3 planes: U^n of the last step (RK save), or null
     double* nfield;      // per-cell Manning n (1 plane), or null
+    double* cbuf;        // 3 planes: third stage buffer of rk3 / rk43, or null
+    double* mpart;       // per-tile flux_h sums of both stages (mass budget), or null
+    long long mpart_cap; // tiles per stage that mpart holds
+    int scheme;          // RKG_TIME_*
     rk::CtrlSlot* ctrl;      // device [2]
     rk::CtrlSlot* ctrl_host; // pinned [2]
     double* mass_buf;    // device [MASS_BLOCKS + 1]
@@ -94,6 +98,8 @@ inline GridDesc make_desc(rkg_grid* g) {
     d.nstrips = (c.nx + WAVE - 1) / WAVE;
     d.nsegs = (c.ny + g->ty - 1) / g->ty;
     d.ctrl = g->ctrl;
+    d.mpart = g->mpart;
+    d.ssp_coeff = g->scheme == RKG_TIME_RK43 ? 2.0 : 1.0;  // rki_timestepper.F90:60-100
     return d;
 }
 

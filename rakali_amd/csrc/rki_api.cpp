@@ -575,7 +575,7 @@ bool validate(const Config& c, std::string& msg) {
 // Features of config_t this build does not run; -5 with a message.
 bool supported(const Config& c, std::string& msg) {
     if (c.grid_type != "structured") { msg = "grid_type 'unstructured' is not implemented here"; return false; }
-    if (c.time_scheme != "rk2") { msg = "time_scheme '" + c.time_scheme + "' is not implemented here (rk2 only)"; return false; }
+    if (c.time_scheme == "semi_implicit") { msg = "time_scheme 'semi_implicit' is not implemented here"; return false; }
     if (c.use_multilayer) { msg = "use_multilayer is not implemented here"; return false; }
     if (c.use_nonhydrostatic) { msg = "use_nonhydrostatic is not implemented here"; return false; }
     if (c.wind_stress_x != 0.0 || c.wind_stress_y != 0.0) { msg = "wind forcing is not implemented here"; return false; }
@@ -587,6 +587,13 @@ bool supported(const Config& c, std::string& msg) {
     if (c.nghost < 2) { msg = "nghost must be >= 2 (the stencil reaches two cells)"; return false; }
     if (c.nx < 2 || c.ny < 2) { msg = "nx and ny must be >= 2 on this path"; return false; }
     return true;
+}
+
+int time_scheme_tag(const std::string& s) {   // parse_time_scheme, rki_timestepper.F90:102-120
+    if (s == "euler") return RKG_TIME_EULER;
+    if (s == "rk3") return RKG_TIME_RK3;
+    if (s == "rk43") return RKG_TIME_RK43;
+    return RKG_TIME_RK2;
 }
 
 int bc_tag(const std::string& s) {   // bc_type_from_string, rki_boundary_types.F90:97-125
@@ -1111,6 +1118,7 @@ int rki_finalize(void* handle) {
     const rkg_config c = device_config(*s);
     int rc = rkg_rc(rkg_create(&c, nullptr, &s->dev), "rkg_create");
     if (!rc) rc = rkg_rc(rkg_set_rk_save(s->dev, 1), "rkg_set_rk_save");
+    if (!rc) rc = rkg_rc(rkg_set_time_scheme(s->dev, time_scheme_tag(s->cfg.time_scheme)), "rkg_set_time_scheme");
     if (!rc) rc = rkg_rc(rkg_set_clock(s->dev, s->t_current, 0), "rkg_set_clock");
     if (!rc) rc = upload_all(*s);
     if (rc) {

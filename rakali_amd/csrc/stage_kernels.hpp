@@ -28,7 +28,7 @@ __device__ __forceinline__ double step_dt(const GridDesc& d, const CtrlSlot& cp,
             dt = d.cfl / max_speed;
         else
             dt = d.cfl * dmin(d.dx, d.dy) / sqrt(d.g * DRY_TOLERANCE);
-        dt = dt * 1.0;  // ssp_coeff of RK2
+        dt = dt * d.ssp_coeff;  // rki_solver.F90:736-741
         return dmin(dt, dt_max);
     }
     return cp.dt;
@@ -58,14 +58,17 @@ __device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, i
 
 // One tile of one stage of one grid.  `where(d)` gives the tile as
 // (strip, segment, leader); the leader is the single tile per grid and
-// launch that advances the control slot.
+// launch that advances the control slot.  STAGE 1 and 2 are the fused
+// SSP-RK2 stages; STAGE 3 is one stage `gs` of a generic SSP scheme.
 template <int STAGE, bool HLLC, bool NF, bool REC, class Where>
 __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p, int mode, double tlim,
-                                           Where where) {
+                                           Where where, const GenStage& gs = GenStage{}) {
     const int3 w = where(d);
     const bool leader = w.z != 0;
     StageCtx s;
-    if (STAGE == 1) {
+    s.alpha = 0.0; s.a_prev = 1.0;
+    s.blend = false; s.first = false; s.last = false;
+    if (STAGE == 1 || (STAGE == 3 && gs.first)) {
         const CtrlSlot cp = d.ctrl[p];
         const bool active = (mode == MODE_STEP) || (cp.t < tlim - 1.0e-10);
         const double dt_max = tlim - cp.t;
@@ -76,13 +79,28 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
             q->dt = dt;
             q->step_count = active ? cp.step_count + 1 : cp.step_count;
             q->speed_bits = active ? 0ull : cp.speed_bits;
+            q->mass_out = cp.mass_out;
             q->active = active ? 1 : 0;
         }
         if (!active) return;
-        s.ih = d.ah; s.ihu = d.ahu; s.ihv = d.ahv;
-        s.oh = d.bh; s.ohu = d.bhu; s.ohv = d.bhv;
-        s.dt = dt;
-        s.eta_tidal = tidal_eta(d, cp.t);
+        if (STAGE == 1) {
+            s.ih = d.ah; s.ihu = d.ahu; s.ihv = d.ahv;
+            s.oh = d.bh; s.ohu = d.bhu; s.ohv = d.bhv;
+            s.dt = dt;
+        } else {
+            s.ih = gs.ih; s.ihu = gs.ihu; s.ihv = gs.ihv;
+            s.oh = gs.oh; s.ohu = gs.ohu; s.ohv = gs.ohv;
+            s.dt = gs.beta * dt;
+        }
+        s.eta_tidal = tidal_eta(d, cp.t);  // stage abscissa c_1 = 0
+    } else if (STAGE == 3) {
+        const CtrlSlot cp = d.ctrl[p];
+        const CtrlSlot cq = d.ctrl[1 - p];
+        if (!cq.active) return;
+        s.ih = gs.ih; s.ihu = gs.ihu; s.ihv = gs.ihv;
+        s.oh = gs.oh; s.ohu = gs.ohu; s.ohv = gs.ohv;
+        s.dt = gs.beta * cq.dt;
+        s.eta_tidal = tidal_eta(d, cp.t + gs.c * cq.dt);  // bc%time = t_base + c_i dt
     } else {
         const CtrlSlot cq = d.ctrl[1 - p];
         if (!cq.active) return;
@@ -91,19 +109,31 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
         s.dt = cq.dt;
         s.eta_tidal = tidal_eta(d, cq.t);  // bc%time = t + dt
     }
+    if (STAGE == 3) {
+        s.alpha = gs.alpha;
+        s.a_prev = 1.0 - gs.alpha;
+        s.blend = gs.alpha != 0.0;
+        s.first = gs.first != 0;
+        s.last = gs.last != 0;
+    }
     s.src = make_src(d, s.dt);
     s.rdx = recip(d.dx);
     s.rdy = recip(d.dy);
     const int strip = w.x, seg = w.y;
-    double smax;
+    double smax, fsum;
     if (tile_is_interior(d, strip, seg))
-        smax = process_tile<STAGE, HLLC, false, NF, REC>(d, s, L, strip, seg);
+        smax = process_tile<STAGE, HLLC, false, NF, REC>(d, s, L, strip, seg, fsum);
     else
-        smax = process_tile<STAGE, HLLC, true, NF, REC>(d, s, L, strip, seg);
-    if (STAGE == 2) {
+        smax = process_tile<STAGE, HLLC, true, NF, REC>(d, s, L, strip, seg, fsum);
+    if (STAGE == 2 || STAGE == 3) {
         smax = wave_max(smax);
         if (threadIdx.x == 0 && smax > 0.0)
             atomicMax(&d.ctrl[1 - p].speed_bits, (unsigned long long)__double_as_longlong(smax));
+    }
+    if (STAGE != 3 && d.mpart != nullptr) {  // accumulate_boundary_mass_out's sum, per tile
+        fsum = wave_sum(fsum);
+        if (threadIdx.x == 0)
+            d.mpart[(long long)(STAGE - 1) * d.nstrips * d.nsegs + (long long)seg * d.nstrips + strip] = fsum;
     }
 }
 
@@ -117,15 +147,81 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_
     });
 }
 
-// Resets control slot p to time t / step count n with an empty speed max.
-__global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long long n) {
+// One stage of a generic SSP scheme, single grid.
+template <bool HLLC, bool NF>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage_gen(GridDesc d, GenStage gs, int p, int mode, double tlim) {
+    __shared__ TileLds L;
+    stage_tile<3, HLLC, NF, false>(d, L, p, mode, tlim, [](const GridDesc& g) {
+        const int tile = xcd_tile(blockIdx.x, g.nstrips * g.nsegs);
+        return make_int3(tile % g.nstrips, tile / g.nstrips, blockIdx.x == 0);
+    }, gs);
+}
+
+// Resets control slot p to time t / step count n with an empty speed max;
+// the mass budget is set to mass_out.
+__global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long long n, double mass_out) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctrl[p].t = t;
         ctrl[p].dt = dt;
         ctrl[p].step_count = n;
         ctrl[p].speed_bits = 0ull;
+        ctrl[p].mass_out = mass_out;
         ctrl[p].active = 1;
     }
+}
+
+// Boundary mass budget of one SSP-RK2 step (rki_solver.F90:507-537, called
+// after each stage's flux at :769 and :796): mass_out -= 0.5 dt sum(flux_h)
+// area per stage.  The per-tile sums of both stages are added in a fixed
+// order (strided per thread, then a tree).  Slot 1-p is the step's result.
+__global__ __launch_bounds__(256) void k_budget(CtrlSlot* ctrl, int p, const double* mpart, int ntiles,
+                                                double area) {
+    __shared__ double red[2][256];
+    if (!ctrl[1 - p].active) return;
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = threadIdx.x; i < ntiles; i += 256) {
+        a1 = a1 + mpart[i];
+        a2 = a2 + mpart[ntiles + i];
+    }
+    red[0][threadIdx.x] = a1;
+    red[1][threadIdx.x] = a2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] = red[0][threadIdx.x] + red[0][threadIdx.x + s];
+            red[1][threadIdx.x] = red[1][threadIdx.x] + red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        CtrlSlot* q = &ctrl[1 - p];
+        const double dt = q->dt;
+        double m = q->mass_out;
+        m = m - 0.5 * dt * red[0][0] * area;
+        m = m - 0.5 * dt * red[1][0] * area;
+        q->mass_out = m;
+    }
+}
+
+// End of a one-stage (Euler) step: the stage wrote B, so B is copied to A
+// and the CFL max of the new state goes to slot 1-p.  (The stage kernel
+// cannot take that max itself: its leader clears the slot's max while the
+// other tiles of the same launch run.)
+__global__ __launch_bounds__(256) void k_euler_finish(GridDesc d, int p) {
+    if (!d.ctrl[1 - p].active) return;
+    double m = 0.0;
+    const Recip rdx = recip(d.dx), rdy = recip(d.dy);
+    const long long n = (long long)d.nx * d.ny;
+    for (long long k = blockIdx.x * 256ll + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+        const int j = (int)(k / d.nx), i = (int)(k % d.nx);
+        const long long o = (long long)j * d.pitch + i;
+        const double h = d.bh[o], hu = d.bhu[o], hv = d.bhv[o];
+        d.ah[o] = h; d.ahu[o] = hu; d.ahv[o] = hv;
+        m = dmax(m, cfl_speed<true>(d.g, rdx, rdy, h, hu, hv));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && m > 0.0)
+        atomicMax(&d.ctrl[1 - p].speed_bits, (unsigned long long)__double_as_longlong(m));
 }
 
 // Max CFL speed of the state A into slot p (compute_timestep's reduction,

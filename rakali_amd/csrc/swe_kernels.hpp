@@ -25,8 +25,9 @@ struct alignas(64) CtrlSlot {
     double dt;                    // dt of the step that produced this slot
     unsigned long long speed_bits;// max CFL speed of the state (as bits)
     long long step_count;         // state%step_count
+    double mass_out;              // boundary mass budget (rki_solver.F90:507-537)
     int active;                   // 1 if the step that wrote this slot ran
-    int pad[5];
+    int pad[3];
 };
 
 struct GridDesc {
@@ -40,7 +41,9 @@ struct GridDesc {
     const double* bed;
     const double* nfield;         // per-cell Manning n, or null (scalar manning_n)
     double* rec;                  // 3 planes receiving each cell's flux divergence, or null
+    double* mpart;                // per-tile sums of flux_h, [stage 1 tiles | stage 2 tiles], or null
     long long plane;              // elements per plane (pitch * ny)
+    double ssp_coeff;             // stable-dt multiplier of the time scheme (rki_timestepper.F90)
     double g, cfl, manning_n, coriolis_f;
     int use_hllc;
     int bc[4];                    // west, east, south, north
@@ -204,6 +207,14 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Butterfly sum: every lane ends with the same bits (each pairwise add sees
+// the same two operands), in an order fixed by the lane numbering.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, WAVE);
+    return v;
+}
+
 // Per-segment data of the two edge columns of a strip, one row per entry:
 // raw cells c0-1 and c0+64, the high x-state of c0-1, and the x-face
 // between c0+63 and c0+64.
@@ -221,6 +232,20 @@ struct StageCtx {
     double eta_tidal;
     Recip rdx, rdy;
     SourceTerms src;
+    // generic SSP stage only (STAGE 3)
+    double alpha, a_prev;         // blend weights on U^n and on the Euler iterate
+    bool blend, first, last;
+};
+
+// One stage of a generic explicit SSP scheme (rki_solver.F90:861-925):
+// E = U^(i-1) + beta_i dt L(U^(i-1)), then U^(i) = alpha_i U^n +
+// (1 - alpha_i) E when alpha_i != 0.  U^n stays in A until the last stage,
+// which writes A.
+struct GenStage {
+    const double* ih; const double* ihu; const double* ihv;
+    double* oh; double* ohu; double* ohv;
+    double alpha, beta, c;        // rki_timestepper.F90:53-58
+    int first, last;
 };
 
 template <bool EDGE>
@@ -249,7 +274,7 @@ struct ExactTag {
 // and writes U^{n+1} over A, returning the lane's max CFL speed.
 template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC>
 __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx& s, TileLds& L,
-                                               int strip, int seg) {
+                                               int strip, int seg, double& fsum) {
     const int lane = threadIdx.x;
     const int c0 = strip * WAVE;
     const int c = c0 + lane;
@@ -259,6 +284,8 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     const double g = d.g;
     const long long pitch = d.pitch;
     double smax = 0.0;
+    double facc = 0.0;  // sum of the lane's flux_h, rows in order
+    const bool read_un = STAGE == 2 || (STAGE == 3 && s.blend);
 
     // (1) edge columns of the segment, one row per lane
     bool edge_tiny = false;
@@ -313,7 +340,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     const bool own = c < d.nx;
     Cell un;  // U^n at (c, j) for the RK2 average
     un.b = 0.0;
-    if (STAGE == 2 && own) {
+    if (read_un && own) {
         const long long k = (long long)j0 * pitch + c;
         un.h = d.ah[k]; un.hu = d.ahu[k]; un.hv = d.ahv[k];
     }
@@ -325,7 +352,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         Cell nxt, nun;
         if (more) {
             nxt = ld<EDGE>(d, s, c, j + 3);
-            if (STAGE == 2 && own) {
+            if (read_un && own) {
                 const long long k = (long long)(j + 1) * pitch + c;
                 nun.h = d.ah[k]; nun.hu = d.ahu[k]; nun.hv = d.ahv[k];
             }
@@ -356,6 +383,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
             if (own) {
                 double fh, fhu, fhv;
                 divergence<EX>(g, s.rdx, s.rdy, fxl, fxr, fyb, fyt, fh, fhu, fhv);
+                facc = facc + fh;
                 const long long k = (long long)j * pitch + c;
                 if (REC) {
                     d.rec[k] = fh; d.rec[d.plane + k] = fhu; d.rec[2 * d.plane + k] = fhv;
@@ -373,13 +401,23 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                     clamp_desing<EX>(g, h, hu, hv);
                     smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
                 }
+                if (STAGE == 3) {
+                    if (s.blend) {  // compute_ssp_combine, rki_kernel_update.F90:183-206
+                        h = s.alpha * un.h + s.a_prev * h;
+                        hu = s.alpha * un.hu + s.a_prev * hu;
+                        hv = s.alpha * un.hv + s.a_prev * hv;
+                        clamp_desing<EX>(g, h, hu, hv);
+                    }
+                    if (s.last) smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
+                }
                 s.oh[k] = h; s.ohu[k] = hu; s.ohv[k] = hv;
-                if (STAGE == 1 && d.sh != nullptr) {  // h0, hu0, hv0 (rki_kernel_fused.F90:224-226)
+                if ((STAGE == 1 || (STAGE == 3 && s.first)) && d.sh != nullptr) {
+                    // h0, hu0, hv0 (rki_kernel_fused.F90:224-226, rki_solver.F90:879-881)
                     d.sh[k] = rc.h; d.shu[k] = rc.hu; d.shv[k] = rc.hv;
                 }
             }
         };
-        const bool lane_tiny = t_prev || t_rc || t_rn || t_rnn || (STAGE == 2 && own && tiny(un));
+        const bool lane_tiny = t_prev || t_rc || t_rn || t_rnn || (read_un && own && tiny(un));
         if (__any(lane_tiny) || ((edge_rows >> r) & 1ull))
             row(ExactTag<true>{});
         else
@@ -395,9 +433,10 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         if (more) {
             rnn = nxt;
             t_rnn = tiny(nxt);
-            if (STAGE == 2) un = nun;
+            if (read_un) un = nun;
         }
     }
+    fsum = facc;
     return smax;
 }
 

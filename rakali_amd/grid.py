@@ -59,15 +59,19 @@ class Grid:
     """One structured grid resident in HBM (fields h, hu, hv, b)."""
 
     def __init__(self, nx: int, ny: int, dx: float, dy: float, *, stream: Optional[int] = None,
-                 **kw):
+                 time_scheme: str = "rk2", **kw):
         self._lib = _lib.load()
         self.config = make_config(nx, ny, dx, dy, **kw)
         self.nx, self.ny = int(nx), int(ny)
+        if time_scheme not in _lib.TIME_SCHEMES:
+            raise ValueError(f"unsupported time scheme {time_scheme!r}")
         h = ctypes.c_void_p()
         check("rkg_create", self._lib.rkg_create(ctypes.byref(self.config),
                                                   ctypes.c_void_p(stream) if stream else None,
                                                   ctypes.byref(h)))
         self._h = h
+        if time_scheme != "rk2":
+            self.time_scheme = time_scheme
 
     # -- lifecycle -----------------------------------------------------
     def close(self):
@@ -110,6 +114,35 @@ class Grid:
     def use_manning_field(self, enable: bool = True) -> None:
         """Per-cell Manning n (field "n", initialised to the scalar value)."""
         check("rkg_use_manning_field", self._lib.rkg_use_manning_field(self.handle, int(bool(enable))))
+
+    @property
+    def time_scheme(self) -> str:
+        """euler, rk2 (the fused SSP-RK2 path), rk3 or rk43."""
+        v = ctypes.c_int()
+        check("rkg_get_time_scheme", self._lib.rkg_get_time_scheme(self.handle, ctypes.byref(v)))
+        return {n: k for k, n in _lib.TIME_SCHEMES.items()}[v.value]
+
+    @time_scheme.setter
+    def time_scheme(self, name: str) -> None:
+        if name not in _lib.TIME_SCHEMES:
+            raise ValueError(f"unsupported time scheme {name!r}")
+        check("rkg_set_time_scheme", self._lib.rkg_set_time_scheme(self.handle, _lib.TIME_SCHEMES[name]))
+
+    def keep_rk_save(self, enable: bool = True) -> None:
+        """Keep U^n of each step (fields h0, hu0, hv0), as state%barotropic%h0."""
+        check("rkg_set_rk_save", self._lib.rkg_set_rk_save(self.handle, int(bool(enable))))
+
+    def track_mass_budget(self, enable: bool = True) -> None:
+        """Accumulate the boundary mass budget (``mass_out``) from 0."""
+        check("rkg_set_mass_budget", self._lib.rkg_set_mass_budget(self.handle, int(bool(enable))))
+
+    @property
+    def mass_out(self) -> float:
+        """Volume that left through the boundary since the budget was enabled
+        (rki_solver.F90:507-537; positive = outflow)."""
+        m = ctypes.c_double()
+        check("rkg_get_mass_budget", self._lib.rkg_get_mass_budget(self.handle, ctypes.byref(m)))
+        return m.value
 
     def device_ptr(self, name: str):
         p = ctypes.c_void_p()

@@ -94,7 +94,7 @@ def test_namelist_syntax_forms(L):
     ("&boundary_nml\n tidal_amp = 1,2,3,4,5,6,7,8,9,10,11\n/", -2, "at most 10"),
     ("&nonhydrostatic_nml\n use_multilayer = .true.\n/", -5, "use_multilayer"),
     ("&nonhydrostatic_nml\n use_nonhydrostatic = .true.\n/", -5, "nonhydrostatic"),
-    ("&time_nml\n time_scheme = 'rk3'\n/", -5, "rk2 only"),
+    ("&time_nml\n time_scheme = 'semi_implicit'\n/", -5, "semi_implicit"),
     ("&boundary_nml\n bc_north = 'chapman'\n/", -5, "chapman"),
     ("&boundary_nml\n bc_north = 'sponge', sponge_width = 4\n/", -5, "sponge"),
     ("&physics_nml\n wind_stress_x = 0.1\n/", -5, "wind"),
