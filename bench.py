@@ -127,7 +127,7 @@ def main():
                         "walls elsewhere), dx=dy=20 m, 64 seeded bumps, Manning n=0.025, HLLC, CFL 0.45",
             "nx": scn.nx, "ny": scn.ny, "dx": scn.dx,
             "dry_fraction_initial": round(scn.dry_fraction(), 4),
-            "tile_rows": args.tile_rows or 32,
+            "tile_rows": args.tile_rows or 16,
             "simulated_time_s": t_sim, "last_dt_s": dt, "total_steps": nsteps, "mass": mass,
         },
         "roofline": {

@@ -146,7 +146,7 @@ int rkg_total_mass(rkg_grid *g, double *mass);
 /* Waits for all queued work of this grid. */
 int rkg_sync(rkg_grid *g);
 
-/* Rows per tile segment of the stage kernels (1..64; default 32). */
+/* Rows per tile segment of the stage kernels (1..64; default 16). */
 int rkg_set_tile_rows(rkg_grid *g, int rows);
 int rkg_get_tile_rows(rkg_grid *g, int *rows);
 /* Keeps U^n of every step in three extra planes (24 B per cell per step of

@@ -16,7 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "crmath.hpp"
-#include "swe_core.hpp"
+#include "swe_kernels.hpp"
 
 namespace rk {
 namespace adj {
@@ -356,6 +356,7 @@ struct BodyRev {
     S3 lv;      // adjoint of the stage input V at the cell
     S3 ldiv;    // adjoint of the flux divergence at the cell
     double ln;  // adjoint of the cell's Manning n
+    double ldt; // adjoint of the stage's dt (this cell's share)
 };
 
 __host__ __device__ __forceinline__ void body_fwd(const SourceTerms& s, double dt, const S3& div, double n_sq,
@@ -389,7 +390,7 @@ __host__ __device__ __forceinline__ BodyRev body_rev(const SourceTerms& s, doubl
     double h = ph, hu = phu, hv = phv;
     clamp_fwd(s.g, h, hu, hv);
     S3 l = lw;
-    double ln = 0.0;
+    double ln = 0.0, ldt = 0.0;
     if (h > DRY_TOLERANCE) {
         double fhu = hu, fhv = hv;   // after friction
         double D = 1.0, u = 0.0, v = 0.0, speed = 0.0, cf = 0.0, h13 = 1.0;
@@ -404,12 +405,16 @@ __host__ __device__ __forceinline__ BodyRev body_rev(const SourceTerms& s, doubl
             fhv = hv / D;
         }
         if (s.has_coriolis) {
+            // rotation by the angle f dt: d(out)/d dt = f (hv_out, -hu_out)
+            const double ohu = fhu * s.cosf + fhv * s.sinf, ohv = -fhu * s.sinf + fhv * s.cosf;
+            ldt += s.coriolis_f * (l.hu * ohv - l.hv * ohu);
             const double a = l.hu, b = l.hv;
             l.hu = a * s.cosf - b * s.sinf;
             l.hv = a * s.sinf + b * s.cosf;
         }
         if (friction) {
             const double lD = -(l.hu * fhu + l.hv * fhv) / D;
+            ldt += lD * (cf * speed) / h;
             l.hu = l.hu / D;
             l.hv = l.hv / D;
             const double X = (dt * cf * speed) / h;
@@ -426,11 +431,67 @@ __host__ __device__ __forceinline__ BodyRev body_rev(const SourceTerms& s, doubl
         }
     }
     const S3 lp = clamp_rev(s.g, ph, phu, phv, l);
+    ldt += lp.h * div.h + lp.hu * div.hu + lp.hv * div.hv;
     BodyRev r;
     r.lv = lp;
     r.ldiv = S3{dt * lp.h, dt * lp.hu, dt * lp.hv};
     r.ln = ln;
+    r.ldt = ldt;
     return r;
+}
+
+// d eta_tidal / d t of tidal_eta (rki_boundary.F90:92-101).
+__host__ __device__ __forceinline__ double tidal_deta(const GridDesc& d, double time) {
+    double e = 0.0;
+    if (d.n_tc > 0) {
+        for (int n = 0; n < d.n_tc; ++n)
+            e = e - d.tc_amp[n] * d.tc_omega[n] * crm::sin(d.tc_omega[n] * time - d.tc_phase[n]);
+    } else if (d.tidal_period > 0.0) {
+        const double w = TWO_PI / d.tidal_period;
+        e = d.tidal_amplitude * w * crm::cos(TWO_PI * time / d.tidal_period);
+    }
+    return e;
+}
+
+// CFL speed of one cell (compute_timestep's term, rki_kernel_timestep.F90
+// :56-69) with IEEE division: the same value as the stage kernels' max.
+__host__ __device__ __forceinline__ double cfl_speed_fwd(double g, double dx, double dy, double h, double hu,
+                                                         double hv) {
+    const double eps4 = 5.062499999999998e-16;
+    if (!(h > DRY_TOLERANCE)) return 0.0;
+    const double ws = sqrt(g * h);
+    const double S = sqrt(((h * h) * h) * h + eps4);
+    const double ua = (fabs(hu) * SQRT2 * h) / S;
+    const double va = (fabs(hv) * SQRT2 * h) / S;
+    const double sx = (ua + ws) / dx;
+    const double sy = (va + ws) / dy;
+    return sx > sy ? sx : sy;
+}
+
+// Adjoint of cfl_speed_fwd: ls is the adjoint of the speed; the branch of
+// the max taken in the forward run.
+__host__ __device__ __forceinline__ S3 cfl_speed_rev(double g, double dx, double dy, double h, double hu, double hv,
+                                                     double ls) {
+    const double eps4 = 5.062499999999998e-16;
+    S3 l{0.0, 0.0, 0.0};
+    if (!(h > DRY_TOLERANCE)) return l;
+    const double ws = sqrt(g * h);
+    const double h4 = ((h * h) * h) * h;
+    const double S = sqrt(h4 + eps4);
+    const double ua = (fabs(hu) * SQRT2 * h) / S;
+    const double va = (fabs(hv) * SQRT2 * h) / S;
+    const double sx = (ua + ws) / dx;
+    const double sy = (va + ws) / dy;
+    const bool xs = sx > sy;
+    const double lsum = xs ? ls / dx : ls / dy;        // adjoint of (u_abs + ws)
+    const double m = xs ? hu : hv;
+    // u_abs = |m| sqrt2 h / S, S = sqrt(h^4 + eps^4): d/dh = |m| sqrt2 (S^2 - 2 h^4) / S^3
+    const double dm = (m > 0.0 ? 1.0 : (m < 0.0 ? -1.0 : 0.0)) * SQRT2 * h / S;
+    const double dh = fabs(m) * SQRT2 * ((h4 + eps4) - 2.0 * h4) / (S * S * S);
+    l.h = lsum * (dh + g / (2.0 * ws));
+    if (xs) l.hu = lsum * dm;
+    else l.hv = lsum * dm;
+    return l;
 }
 
 // ------------------------------------------------------------- boundaries
@@ -439,7 +500,7 @@ __host__ __device__ __forceinline__ BodyRev body_rev(const SourceTerms& s, doubl
 __host__ __device__ __forceinline__ void ghost_rev(int tag, double g, double eta_tidal, double inflow_velocity,
                                                    double clamped_elevation, const Cell& mir, const Cell& in0,
                                                    double nrm_sign, bool normal_is_u, const C4& lg, C4& lmir,
-                                                   C4& lin0) {
+                                                   C4& lin0, double* leta = nullptr) {
     const double lgn = normal_is_u ? lg.hu : lg.hv;
     const double lgt = normal_is_u ? lg.hv : lg.hu;
     double* lin0n = normal_is_u ? &lin0.hu : &lin0.hv;
@@ -478,6 +539,11 @@ __host__ __device__ __forceinline__ void ghost_rev(int tag, double g, double eta
         if (h_int > 1.0e-6) lh_int += lhmax;
         if (prescribe) { if (pre > 0.0) lgb -= lhg; }
         else lh_int += lhg;
+        if (prescribe && leta) {   // d/d eta_ext of h_ghost and of the Flather term
+            double le = -lB * nrm_sign * speed / hmax;
+            if (pre > 0.0) le += lhg;
+            *leta += le;
+        }
         lin0.b += lgb;
         lin0.h += lh_int;
         return;

@@ -15,9 +15,12 @@
 //   k_adj_faces -- tile kernel: every face and slope of the stage, with the
 //                  boundary ghosts, transposed onto the stage input and the
 //                  bed.
-// dt and the boundary times are those of the forward run (held fixed): the
-// gradient is exact for the discrete scheme at fixed time levels, and at
-// fixed dt (dt_fixed > 0) it is the exact gradient of the scheme.
+//   k_dt_rev    -- one workgroup per step: the dependence of the loss on the
+//                  step's dt (update, friction, Coriolis angle, the tide's
+//                  clock t_{s+1} = t_s + dt_s) and, for a CFL dt, on the
+//                  cell that set the max speed (k_argmax_part finds it).
+// The gradient is that of the discrete scheme with its dt rule: fixed dt,
+// the CFL dt every cfl_interval steps, and the t_end bound.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,9 +63,21 @@ struct AdjStage {
     double* lam_div;            // 3 planes
     double* grad_b;
     double* grad_n;
-    const double* hist;         // (t, dt) of the step
+    const double* hist;         // (t, dt, dt recomputed) of the step
+    double* part_dt;            // per body workgroup: sum of d L / d dt of its cells
+    double* edge_lg;            // adjoint (h, normal momentum) of every tidal ghost cell, see edge_slot
+    int edge_len;               // max(nx, ny)
     int ty;                     // rows per tile of k_adj_faces
 };
+
+// Slot of ghost layer kk (0 = next to the edge) at position p along edge e
+// (0 W, 1 E, 2 S, 3 N) in edge_lg (two doubles per slot).
+__host__ __device__ __forceinline__ long long edge_slot(int e, int kk, int p, int len) {
+    return ((long long)(e * 2 + kk) * len + p) * 2;
+}
+
+constexpr int HIST = 3;         // doubles per step in hist
+constexpr int ARGMAX_BLOCKS = 1024;
 
 __device__ __forceinline__ SourceTerms adj_src(const GridDesc& d, double dt) {
     SourceTerms s;
@@ -72,11 +87,24 @@ __device__ __forceinline__ SourceTerms adj_src(const GridDesc& d, double dt) {
     s.has_coriolis = fabs(d.coriolis_f) > 1.0e-20;
     s.cosf = 0.0;
     s.sinf = 0.0;
+    s.coriolis_f = d.coriolis_f;
     if (s.has_coriolis) {
         s.cosf = crm::cos(d.coriolis_f * dt);
         s.sinf = crm::sin(d.coriolis_f * dt);
     }
     return s;
+}
+
+// Sum over a 256-thread workgroup in a fixed order; the result is valid in
+// thread 0.
+__device__ __forceinline__ double block_sum256(double v, double* red4) {
+    v = wave_sum(v);
+    if ((threadIdx.x & (WAVE - 1)) == 0) red4[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = ((red4[0] + red4[1]) + red4[2]) + red4[3];
+    __syncthreads();
+    return r;
 }
 
 // Cell-local reverse of one stage.  STAGE 2: lam_out holds the adjoint of
@@ -90,6 +118,7 @@ __global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
     const SourceTerms src = adj_src(d, dt);
     const long long P = d.plane;
     const long long n = (long long)d.nx * d.ny;
+    double ldt = 0.0;
     for (long long idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
         const int j = (int)(idx / d.nx), i = (int)(idx % d.nx);
         const long long k = (long long)j * d.pitch + i;
@@ -118,7 +147,11 @@ __global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
         }
         a.lam_div[k] = r.ldiv.h; a.lam_div[P + k] = r.ldiv.hu; a.lam_div[2 * P + k] = r.ldiv.hv;
         if (d.nfield && a.grad_n) a.grad_n[k] += r.ln;
+        ldt = ldt + r.ldt;
     }
+    __shared__ double red4[4];
+    const double tot = block_sum256(ldt, red4);
+    if (threadIdx.x == 0) a.part_dt[blockIdx.x] = tot;
 }
 
 struct Acc {   // adjoint accumulator of one cell: (h, hu, hv, b)
@@ -186,7 +219,9 @@ constexpr int NWIN = 6;   // accumulator rows r-3 .. r+2
 // r and r+1, the y slope of row r and, for an owned row, its x faces and
 // x slopes.  Row r-1 is complete after iteration r; rows are written two
 // iterations later so that ghost rows can still be folded into them.
-template <bool HLLC, int STAGE>
+// TX / TY: a west or east / a south or north edge is tidal, and the
+// adjoints of its ghost cells are kept in edge_lg for k_eta_rev.
+template <bool HLLC, int STAGE, bool TX, bool TY>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_faces(AdjStage a) {
     const GridDesc& d = a.d;
     const double tau = STAGE == 1 ? a.hist[0] : a.hist[0] + a.hist[1];   // bc%time of the stage
@@ -225,6 +260,15 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) vo
                 int mrow, irow;
                 adj::C4 lm, li;
                 ghost_y_rev(d, eta, x, rho, lg, mrow, irow, lm, li);
+                // a tidal ghost row cell is kept once, by the tile that owns its
+                // column and the edge row (a tile next to a short last tile also
+                // reaches the ghost rows, with partial sums)
+                const int e = rho < 0 ? 2 : 3;
+                if (TY && own_col && (rho < 0 ? j0 == 0 : j1 == d.ny) && d.bc[e] == BC_TIDAL) {
+                    const long long o = edge_slot(e, rho < 0 ? -rho - 1 : rho - d.ny, x, a.edge_len);
+                    a.edge_lg[o] = lg.h;
+                    a.edge_lg[o + 1] = lg.hv;
+                }
                 // source rows are in the window: slot of row t is t - (r - 3)
                 const int qm = mrow - (r - 3), qi = irow - (r - 3);
 #pragma unroll
@@ -241,7 +285,15 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) vo
         if (c0 - 2 < 0 || c0 + WAVE - 3 >= d.nx) {
             adj::C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
             int mcol = -100, icol = -100;
-            if (x == -1 || x == -2 || x == d.nx || x == d.nx + 1) ghost_x_rev(d, eta, x, rho, lg, mcol, icol, lm, li);
+            if (x == -1 || x == -2 || x == d.nx || x == d.nx + 1) {
+                ghost_x_rev(d, eta, x, rho, lg, mcol, icol, lm, li);
+                const int e = x < 0 ? 0 : 1;
+                if (TX && rho >= j0 && rho < j1 && (x < 0 ? c0 == 0 : c1 == d.nx) && d.bc[e] == BC_TIDAL) {
+                    const long long o = edge_slot(e, x < 0 ? -x - 1 : x - d.nx, rho, a.edge_len);
+                    a.edge_lg[o] = lg.h;
+                    a.edge_lg[o + 1] = lg.hu;
+                }
+            }
             // lanes holding columns -2, -1, nx, nx + 1 (when inside the wavefront)
             const int srcs[4] = {-c0, 1 - c0, d.nx - c0 + 2, d.nx - c0 + 3};
 #pragma unroll
@@ -340,6 +392,44 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) vo
     }
 }
 
+// d L / d eta_tidal of one stage from its tidal ghost cells (edge_lg), in
+// a fixed order: the eta_ext terms of bc_flather_edge (rki_boundary.F90
+// :331-343), h_g = max(eta_ext - b_g, 0) and the radiation term.
+__global__ __launch_bounds__(256) void k_eta_rev(GridDesc d, const double* hist, int stage, const double* edge_lg,
+                                                 int len, double* out) {
+    __shared__ double red[256];
+    const double eta = tidal_eta(d, stage == 1 ? hist[0] : hist[0] + hist[1]);
+    double acc = 0.0;
+    for (int e = 0; e < 4; ++e) {
+        if (d.bc[e] != BC_TIDAL) continue;
+        const bool xe = e < 2;
+        const int n = xe ? d.ny : d.nx;
+        const double sgn = (e == 0 || e == 2) ? -1.0 : 1.0;
+        for (int i = threadIdx.x; i < 2 * n; i += 256) {
+            const int kk = i / n, p = i % n;
+            const int ci = xe ? (e == 0 ? 0 : d.nx - 1) : p;
+            const int cj = xe ? p : (e == 2 ? 0 : d.ny - 1);
+            const Cell in0 = load_direct(d.ah, d.ahu, d.ahv, d.bed, d.pitch, ci, cj);
+            const long long o = edge_slot(e, kk, p, len);
+            adj::C4 lg{edge_lg[o], 0.0, 0.0, 0.0};
+            if (xe) lg.hu = edge_lg[o + 1];
+            else lg.hv = edge_lg[o + 1];
+            adj::C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
+            double le = 0.0;
+            adj::ghost_rev(BC_TIDAL, d.g, eta, d.inflow_velocity, d.clamped_elevation, in0, in0, sgn, xe, lg, lm, li,
+                           &le);
+            acc = acc + le;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
 // Gauge misfit of one state: loss += sum_k (h + b - target_k)^2, eta
 // recorded per gauge; a single workgroup, fixed summation order.
 __global__ __launch_bounds__(256) void k_gauge_loss(const double* __restrict__ h, const double* __restrict__ bed,
@@ -379,12 +469,112 @@ __global__ void k_gauge_seed(const double* __restrict__ h, const double* __restr
     }
 }
 
-// (t, dt) of the step that starts from control slot p.
-__global__ void k_hist(const CtrlSlot* ctrl, int p, double* out) {
+// (t, dt, dt recomputed) of the step that starts from control slot p
+// (the recompute test of rki_solver.F90:731).
+__global__ void k_hist(GridDesc d, const CtrlSlot* ctrl, int p, double* out) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        out[0] = ctrl[p].t;
+        const CtrlSlot cp = ctrl[p];
+        out[0] = cp.t;
         out[1] = ctrl[1 - p].dt;
+        out[2] = (d.dt_fixed <= 0.0 && (cp.step_count % d.cfl_interval == 0 || cp.dt <= 0.0)) ? 1.0 : 0.0;
     }
+}
+
+// Max CFL speed of a state and the first cell (x fastest) that attains it,
+// per workgroup; a speed of 0 has no cell (-1).
+__global__ __launch_bounds__(256) void k_argmax_part(GridDesc d, double* pval, long long* pidx) {
+    __shared__ double sv[256];
+    __shared__ long long si[256];
+    double best = 0.0;
+    long long bi = -1;
+    const long long n = (long long)d.nx * d.ny;
+    for (long long idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+        const long long k = (idx / d.nx) * d.pitch + idx % d.nx;
+        const double v = adj::cfl_speed_fwd(d.g, d.dx, d.dy, d.ah[k], d.ahu[k], d.ahv[k]);
+        if (v > best) { best = v; bi = idx; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double v = sv[threadIdx.x + s];
+            const long long i = si[threadIdx.x + s];
+            if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && i >= 0 && (si[threadIdx.x] < 0 || i < si[threadIdx.x]))) {
+                sv[threadIdx.x] = v;
+                si[threadIdx.x] = i;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { pval[blockIdx.x] = sv[0]; pidx[blockIdx.x] = si[0]; }
+}
+
+// Scalar part of the reverse of one step (one workgroup).  scal[0] is the
+// adjoint of t_{s+1} carried from later steps, scal[1] that of a dt that
+// the next step reused.  Adds d L / d dt * d dt / d U at the cell of
+// maximum speed to lam (the adjoint of U_s) when dt was the CFL dt.
+__global__ __launch_bounds__(256) void k_dt_rev(GridDesc d, const double* hist, const double* part_dt, int nbody,
+                                                const double* leta, const double* pval, const long long* pidx,
+                                                int nargmax, double* lam, double* scal) {
+    __shared__ double red[256];
+    __shared__ double sv[256];
+    __shared__ long long si[256];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < 2 * nbody; i += 256) a = a + part_dt[i];
+    red[threadIdx.x] = a;
+    double best = 0.0;
+    long long bi = -1;
+    for (int i = threadIdx.x; i < nargmax; i += 256) {
+        const double v = pval[i];
+        const long long x = pidx[i];
+        if (v > best || (v == best && x >= 0 && (bi < 0 || x < bi))) { best = v; bi = x; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+            const double v = sv[threadIdx.x + s];
+            const long long x = si[threadIdx.x + s];
+            if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && x >= 0 && (si[threadIdx.x] < 0 || x < si[threadIdx.x]))) {
+                sv[threadIdx.x] = v;
+                si[threadIdx.x] = x;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double t = hist[0], dt = hist[1];
+    const bool recomputed = hist[2] != 0.0;
+    const double lam_t1 = scal[0] + leta[1] * adj::tidal_deta(d, t + dt);   // stage 2 ran at t + dt
+    const double gdt = red[0] + lam_t1 + scal[1];
+    double lam_t = lam_t1 + leta[0] * adj::tidal_deta(d, t);
+    double carry = 0.0;
+    const double dt_max = d.t_end - t;
+    if (d.dt_fixed > 0.0) {
+        if (!(d.dt_fixed < dt_max)) lam_t -= gdt;           // dt = t_end - t
+    } else if (recomputed) {
+        const double ms = sv[0];
+        double dc = ms > 0.0 ? d.cfl / ms : d.cfl * dmin(d.dx, d.dy) / sqrt(d.g * DRY_TOLERANCE);
+        dc = dc * d.ssp_coeff;
+        if (!(dc < dt_max)) {
+            lam_t -= gdt;                                    // dt = t_end - t
+        } else if (ms > 0.0 && si[0] >= 0) {                 // dt = cfl ssp / max speed
+            const long long idx = si[0];
+            const long long k = (idx / d.nx) * d.pitch + idx % d.nx;
+            const double lms = gdt * (-(d.cfl * d.ssp_coeff) / (ms * ms));
+            const adj::S3 l = adj::cfl_speed_rev(d.g, d.dx, d.dy, d.ah[k], d.ahu[k], d.ahv[k], lms);
+            lam[k] += l.h;
+            lam[d.plane + k] += l.hu;
+            lam[2 * d.plane + k] += l.hv;
+        }
+    } else {
+        carry = gdt;                                         // dt reused from the step before
+    }
+    scal[0] = lam_t;
+    scal[1] = carry;
 }
 
 // Still-water initial state: h = max(eta0 - b, 0) and its bed adjoint.
@@ -413,7 +603,12 @@ struct rkg_adjoint {
     std::vector<int> ckpt_parity;
     double* ws;                 // segment workspace: A (K+1) x 3, B K x 3, div1 K x 3, div2 K x 3
     double* lam;                // lamA (3), lamB (3), lamDiv (3), grad_b, grad_n
-    double* hist;               // K x 2
+    double* hist;               // K x HIST
+    double* part;               // part_dt (2 x body blocks), argmax values, d L / d eta_tidal (2)
+    long long* part_idx;        // argmax cells
+    double* scal;               // adjoint of the clock and of a reused dt
+    double* edge_lg;            // tidal ghost adjoints of one stage (edge_slot)
+    int edge_len;
     double* loss_dev;
     double* loss_host;
     CtrlSlot end_ctrl[2];
@@ -458,6 +653,20 @@ void launch_rec(rkg_grid* g, const GridDesc& d, int p) {
         hipLaunchKernelGGL((k_stage<STAGE, false, true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, p, MODE_STEP, tlim);
 }
 
+template <bool HLLC, int STAGE>
+void launch_faces(rkg_grid* g, const AdjStage& st, int nblocks) {
+    const int* bc = g->cfg.bc;
+    const bool tx = bc[0] == BC_TIDAL || bc[1] == BC_TIDAL, ty = bc[2] == BC_TIDAL || bc[3] == BC_TIDAL;
+    if (tx && ty)
+        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, true, true>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
+    else if (tx)
+        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, true, false>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
+    else if (ty)
+        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, false, true>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
+    else
+        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, false, false>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
+}
+
 int body_blocks(const rkg_grid* g) {
     const long long n = (long long)g->cfg.nx * g->cfg.ny;
     return (int)std::min<long long>((n + 255) / 256, 8192);
@@ -488,6 +697,10 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
                       g->cfg.ny);
             return RKG_ERR_RANGE;
         }
+    if (g->scheme != RKG_TIME_RK2) {
+        set_error("the adjoint is that of the SSP-RK2 step; set time scheme rk2");
+        return RKG_ERR_UNSUPPORTED;
+    }
     int rc = rkg_use_manning_field(g, 1);
     if (rc) return rc;
     rkg_adjoint* a = new rkg_adjoint();
@@ -516,7 +729,14 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
     if (e == hipSuccess) e = hipMalloc((void**)&a->ckpt_ctrl, (size_t)a->nseg * 2 * sizeof(CtrlSlot));
     if (e == hipSuccess) e = hipMalloc((void**)&a->ws, (size_t)(12 * a->K + 3) * Pl * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&a->lam, 11 * Pl * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc((void**)&a->hist, (size_t)a->K * 2 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->hist, (size_t)a->K * HIST * sizeof(double));
+    a->edge_len = std::max(g->cfg.nx, g->cfg.ny);
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&a->part, (size_t)(2 * body_blocks(g) + ARGMAX_BLOCKS + 2) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->edge_lg, (size_t)edge_slot(4, 0, 0, a->edge_len) * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(a->edge_lg, 0, (size_t)edge_slot(4, 0, 0, a->edge_len) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->part_idx, ARGMAX_BLOCKS * sizeof(long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->scal, 2 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&a->loss_dev, sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void**)&a->loss_host, sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemset(a->ws, 0, (size_t)(12 * a->K + 3) * Pl * sizeof(double));
@@ -539,7 +759,8 @@ void rkg_adjoint_destroy(rkg_adjoint** pa) {
     if (a->magic != ADJ_MAGIC) { *pa = nullptr; return; }
     if (a->g && a->g->stream) (void)hipStreamSynchronize(a->g->stream);
     for (void* p : {(void*)a->gi, (void*)a->gj, (void*)a->tgt, (void*)a->eta, (void*)a->ckpt, (void*)a->ckpt_ctrl,
-                    (void*)a->ws, (void*)a->lam, (void*)a->hist, (void*)a->loss_dev})
+                    (void*)a->ws, (void*)a->lam, (void*)a->hist, (void*)a->loss_dev, (void*)a->part,
+                    (void*)a->part_idx, (void*)a->scal, (void*)a->edge_lg})
         if (p) (void)hipFree(p);
     if (a->loss_host) (void)hipHostFree(a->loss_host);
     if (a->ev[0]) (void)hipEventDestroy(a->ev[0]);
@@ -571,6 +792,10 @@ int rkg_adjoint_gauge_eta(rkg_adjoint* a, double* eta) {
 int rkg_adjoint_forward(rkg_adjoint* a, double* loss) {
     if (!valid(a)) return RKG_ERR_BAD_HANDLE;
     rkg_grid* g = a->g;
+    if (g->scheme != RKG_TIME_RK2) {
+        set_error("the adjoint is that of the SSP-RK2 step; set time scheme rk2");
+        return RKG_ERR_UNSUPPORTED;
+    }
     const size_t Pl = g->plane;
     HIP_TRY(hipMemsetAsync(a->loss_dev, 0, sizeof(double), g->stream));
     HIP_TRY(hipEventRecord(a->ev[0], g->stream));
@@ -618,7 +843,15 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
     const size_t Pl = g->plane;
     HIP_TRY(hipEventRecord(a->ev[0], g->stream));
     HIP_TRY(hipMemsetAsync(a->lam, 0, 11 * Pl * sizeof(double), g->stream));
+    HIP_TRY(hipMemsetAsync(a->scal, 0, 2 * sizeof(double), g->stream));
     const int fblocks = faces_tiles(g, tile_rows);
+    const int nb = body_blocks(g);
+    double* part_dt = a->part;
+    double* pval = a->part + 2 * nb;
+    double* leta = a->part + 2 * nb + ARGMAX_BLOCKS;   // stage 1, stage 2
+    const bool cfl_dt = g->cfg.dt_fixed <= 0.0;
+    const long long cells = (long long)g->cfg.nx * g->cfg.ny;
+    const int nargmax = cfl_dt ? (int)std::min<long long>((cells + 255) / 256, ARGMAX_BLOCKS) : 0;
     for (int s = a->nseg - 1; s >= 0; --s) {
         const int n0 = s * a->K, n1 = std::min(a->nsteps, n0 + a->K), Ks = n1 - n0;
         // replay the segment with recording
@@ -632,7 +865,7 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
             d1.rec = wsD1(a, k);
             launch_rec<1>(g, d1, p);
             // after stage 1: slot p holds t of the step, slot 1 - p its dt
-            hipLaunchKernelGGL(k_hist, dim3(1), dim3(64), 0, g->stream, g->ctrl, p, a->hist + 2 * k);
+            hipLaunchKernelGGL(k_hist, dim3(1), dim3(64), 0, g->stream, d1, g->ctrl, p, a->hist + HIST * k);
             GridDesc d2 = d1;
             d2.rec = wsD2(a, k);
             launch_rec<2>(g, d2, p);
@@ -655,24 +888,33 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
             st.lam_div = lamDiv(a);
             st.grad_b = gradB(a);
             st.grad_n = gradN(a);
-            st.hist = a->hist + 2 * k;
+            st.hist = a->hist + HIST * k;
+            st.part_dt = part_dt + nb;
+            st.edge_lg = a->edge_lg;
+            st.edge_len = a->edge_len;
             st.ty = tile_rows;
-            hipLaunchKernelGGL(k_adj_body<2>, dim3(body_blocks(g)), dim3(256), 0, g->stream, st);
-            if (g->cfg.use_hllc)
-                hipLaunchKernelGGL((k_adj_faces<true, 2>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
-            else
-                hipLaunchKernelGGL((k_adj_faces<false, 2>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
+            hipLaunchKernelGGL(k_adj_body<2>, dim3(nb), dim3(256), 0, g->stream, st);
+            if (g->cfg.use_hllc) launch_faces<true, 2>(g, st, fblocks);
+            else launch_faces<false, 2>(g, st, fblocks);
+            hipLaunchKernelGGL(k_eta_rev, dim3(1), dim3(256), 0, g->stream, st.d, st.hist, 2, a->edge_lg, a->edge_len,
+                               leta + 1);
             // stage 1: input U^n (A_k)
             st.d = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
             st.un = nullptr;
             st.div = wsD1(a, k);
             st.lam_in = lamA(a);
             st.lam_out = lamB(a);
-            hipLaunchKernelGGL(k_adj_body<1>, dim3(body_blocks(g)), dim3(256), 0, g->stream, st);
-            if (g->cfg.use_hllc)
-                hipLaunchKernelGGL((k_adj_faces<true, 1>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
-            else
-                hipLaunchKernelGGL((k_adj_faces<false, 1>), dim3(fblocks), dim3(WAVE), 0, g->stream, st);
+            st.part_dt = part_dt;
+            hipLaunchKernelGGL(k_adj_body<1>, dim3(nb), dim3(256), 0, g->stream, st);
+            if (g->cfg.use_hllc) launch_faces<true, 1>(g, st, fblocks);
+            else launch_faces<false, 1>(g, st, fblocks);
+            hipLaunchKernelGGL(k_eta_rev, dim3(1), dim3(256), 0, g->stream, st.d, st.hist, 1, a->edge_lg, a->edge_len,
+                               leta);
+            // the step's dt and clock
+            if (nargmax > 0)
+                hipLaunchKernelGGL(k_argmax_part, dim3(nargmax), dim3(256), 0, g->stream, st.d, pval, a->part_idx);
+            hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(256), 0, g->stream, st.d, st.hist, part_dt, nb, leta, pval,
+                               a->part_idx, nargmax, lamA(a), a->scal);
         }
     }
     HIP_TRY(hipGetLastError());

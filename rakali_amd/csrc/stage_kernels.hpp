@@ -42,6 +42,7 @@ __device__ __forceinline__ SourceTerms make_src(const GridDesc& d, double dt) {
     s.has_coriolis = fabs(d.coriolis_f) > 1.0e-20;
     s.cosf = 0.0;
     s.sinf = 0.0;
+    s.coriolis_f = d.coriolis_f;
     if (s.has_coriolis) {
         s.cosf = crm::cos(d.coriolis_f * dt);
         s.sinf = crm::sin(d.coriolis_f * dt);

@@ -299,6 +299,7 @@ struct SourceTerms {
     int has_friction;   // manning_n > 0
     int has_coriolis;   // |f| > 1e-20
     double cosf, sinf;  // cos(f dt), sin(f dt)
+    double coriolis_f;  // f (used by the reverse sweep's d/d dt)
 };
 
 // Euler update + clamp + Manning + Coriolis, rki_kernel_fused.F90:228-290

@@ -5,6 +5,12 @@
 // by the tests to pin the GPU adjoint cell by cell; it is itself pinned
 // against central differences of its own forward run.
 //
+// dt is fixed (dt > 0, bounded by t_end - t) or the CFL dt of each step
+// (dt <= 0: cfl / max speed every cfl_interval steps, bounded by t_end -
+// t); the reverse sweep carries the dependence of the loss on every dt
+// (through the update, friction, Coriolis and the tide's time) and on the
+// clock, and puts d dt / d U on the cell that set the max speed.
+//
 // extern "C" int adjref_run(...) -- see the signature below.
 #include <cmath>
 #include <cstring>
@@ -25,8 +31,12 @@ struct Cfg {
     int bc[4];
     double tidal_amplitude, tidal_period, coriolis_f;
     double inflow_velocity, discharge_flux, clamped_elevation;
-    double dt;
+    double dt;              // > 0: fixed dt; <= 0: CFL dt
+    double cfl, t_end;
+    int cfl_interval;
 };
+
+enum DtMode { DT_CONST = 0, DT_CFL = 1, DT_CAPPED = 2, DT_REUSED = 3 };
 
 struct State {
     std::vector<double> h, hu, hv;
@@ -86,14 +96,20 @@ double tide(const Cfg& c, double t) {
     return tidal_eta(d, t);
 }
 
-SourceTerms src_of(const Cfg& c) {
+double dtide(const Cfg& c, double t) {
+    GridDesc d = desc_of(c);
+    return tidal_deta(d, t);
+}
+
+SourceTerms src_of(const Cfg& c, double dt) {
     SourceTerms s;
     s.g = c.g;
     s.n_sq = 0.0;
     s.has_friction = 1;
     s.has_coriolis = std::fabs(c.coriolis_f) > 1.0e-20;
-    s.cosf = s.has_coriolis ? crm::cos(c.coriolis_f * c.dt) : 0.0;
-    s.sinf = s.has_coriolis ? crm::sin(c.coriolis_f * c.dt) : 0.0;
+    s.cosf = s.has_coriolis ? crm::cos(c.coriolis_f * dt) : 0.0;
+    s.sinf = s.has_coriolis ? crm::sin(c.coriolis_f * dt) : 0.0;
+    s.coriolis_f = c.coriolis_f;
     return s;
 }
 
@@ -134,17 +150,18 @@ void divergence_all(const Grid& G, const std::vector<Cell>& V, std::vector<doubl
 }
 
 template <bool HLLC>
-void stage_fwd(const Grid& G, const State& in, const State* un, double eta, State& out, std::vector<double> div[3]) {
+void stage_fwd(const Grid& G, const State& in, const State* un, double eta, double dt, State& out,
+               std::vector<double> div[3]) {
     const Cfg& c = G.c;
     const int N = c.nx * c.ny;
     const std::vector<Cell> V = ghosted(G, in, eta);
     for (int q = 0; q < 3; ++q) div[q].assign(N, 0.0);
     divergence_all<HLLC>(G, V, div[0], div[1], div[2]);
-    const SourceTerms s = src_of(c);
+    const SourceTerms s = src_of(c, dt);
     out.h.resize(N); out.hu.resize(N); out.hv.resize(N);
     for (int k = 0; k < N; ++k) {
         double h = in.h[k], hu = in.hu[k], hv = in.hv[k];
-        body_fwd(s, c.dt, S3{div[0][k], div[1][k], div[2][k]}, G.n[k] * G.n[k], G.n[k] > 0.0, h, hu, hv);
+        body_fwd(s, dt, S3{div[0][k], div[1][k], div[2][k]}, G.n[k] * G.n[k], G.n[k] > 0.0, h, hu, hv);
         if (un) {
             h = 0.5 * (un->h[k] + h); hu = 0.5 * (un->hu[k] + hu); hv = 0.5 * (un->hv[k] + hv);
             clamp_fwd(c.g, h, hu, hv);
@@ -157,7 +174,7 @@ void stage_fwd(const Grid& G, const State& in, const State* un, double eta, Stat
 // flux-divergence adjoint ldiv) to lam (h, hu, hv) and gb.
 template <bool HLLC>
 void faces_rev(const Grid& G, const State& in, double eta, const std::vector<double> ldiv[3], State& lam,
-               std::vector<double>& gb) {
+               std::vector<double>& gb, double& leta) {
     const Cfg& c = G.c;
     const double hg = 0.5 * c.g;
     const std::vector<Cell> V = ghosted(G, in, eta);
@@ -211,12 +228,12 @@ void faces_rev(const Grid& G, const State& in, double eta, const std::vector<dou
         for (int kk = 0; kk < 2; ++kk) {
             C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
             ghost_rev(c.bc[0], c.g, eta, c.inflow_velocity, c.clamped_elevation, cell(kk, j), cell(0, j), -1.0, true,
-                      lV[G.gx(-1 - kk, j)], lm, li);
+                      lV[G.gx(-1 - kk, j)], lm, li, &leta);
             addi(kk, j, lm);
             addi(0, j, li);
             C4 rm{0, 0, 0, 0}, ri{0, 0, 0, 0};
             ghost_rev(c.bc[1], c.g, eta, c.inflow_velocity, c.clamped_elevation, cell(c.nx - 1 - kk, j),
-                      cell(c.nx - 1, j), 1.0, true, lV[G.gx(c.nx + kk, j)], rm, ri);
+                      cell(c.nx - 1, j), 1.0, true, lV[G.gx(c.nx + kk, j)], rm, ri, &leta);
             addi(c.nx - 1 - kk, j, rm);
             addi(c.nx - 1, j, ri);
         }
@@ -224,12 +241,12 @@ void faces_rev(const Grid& G, const State& in, double eta, const std::vector<dou
         for (int kk = 0; kk < 2; ++kk) {
             C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
             ghost_rev(c.bc[2], c.g, eta, c.inflow_velocity, c.clamped_elevation, cell(i, kk), cell(i, 0), -1.0, false,
-                      lV[G.gx(i, -1 - kk)], lm, li);
+                      lV[G.gx(i, -1 - kk)], lm, li, &leta);
             addi(i, kk, lm);
             addi(i, 0, li);
             C4 rm{0, 0, 0, 0}, ri{0, 0, 0, 0};
             ghost_rev(c.bc[3], c.g, eta, c.inflow_velocity, c.clamped_elevation, cell(i, c.ny - 1 - kk),
-                      cell(i, c.ny - 1), 1.0, false, lV[G.gx(i, c.ny + kk)], rm, ri);
+                      cell(i, c.ny - 1), 1.0, false, lV[G.gx(i, c.ny + kk)], rm, ri, &leta);
             addi(i, c.ny - 1 - kk, rm);
             addi(i, c.ny - 1, ri);
         }
@@ -252,12 +269,39 @@ int run(const Cfg& c, const double* b, const double* n, const double* h0, const 
     std::vector<std::vector<double>> div1(3 * nsteps), div2(3 * nsteps);
     U[0].h.assign(h0, h0 + N); U[0].hu.assign(hu0, hu0 + N); U[0].hv.assign(hv0, hv0 + N);
     double L = 0.0;
-    std::vector<double> tt(nsteps + 1, 0.0);   // t_{s+1} = t_s + dt, as the step's clock
-    for (int s = 0; s < nsteps; ++s) tt[s + 1] = tt[s] + c.dt;
+    std::vector<double> tt(nsteps + 1, 0.0), dts(nsteps, 0.0), msv(nsteps, 0.0);   // t_{s+1} = t_s + dt_s
+    std::vector<int> mode(nsteps, DT_CONST), amx(nsteps, -1);
+    double prev_dt = 0.0;
     for (int s = 0; s < nsteps; ++s) {
+        // dt of the step: rki_solver.F90:728-745 with dt_max = t_end - t (rki_api.F90:377)
+        const double dt_max = c.t_end - tt[s];
+        double dt;
+        if (c.dt > 0.0) {
+            dt = c.dt < dt_max ? c.dt : dt_max;
+            mode[s] = c.dt < dt_max ? DT_CONST : DT_CAPPED;
+        } else if (s % c.cfl_interval == 0 || prev_dt <= 0.0) {
+            double ms = 0.0;
+            int am = -1;
+            for (int k = 0; k < N; ++k) {
+                const double v = cfl_speed_fwd(c.g, c.dx, c.dy, U[s].h[k], U[s].hu[k], U[s].hv[k]);
+                if (v > ms) { ms = v; am = k; }
+            }
+            double dc = ms > 0.0 ? c.cfl / ms : c.cfl * std::min(c.dx, c.dy) / std::sqrt(c.g * DRY_TOLERANCE);
+            dc = dc * 1.0;
+            dt = dc < dt_max ? dc : dt_max;
+            mode[s] = dc < dt_max ? (ms > 0.0 ? DT_CFL : DT_CONST) : DT_CAPPED;
+            msv[s] = ms;
+            amx[s] = am;
+        } else {
+            dt = prev_dt;
+            mode[s] = DT_REUSED;
+        }
+        prev_dt = dt;
+        dts[s] = dt;
+        tt[s + 1] = tt[s] + dt;
         const double t = tt[s];
-        stage_fwd<HLLC>(G, U[s], nullptr, tide(c, t), Us[s], &div1[3 * s]);
-        stage_fwd<HLLC>(G, Us[s], &U[s], tide(c, t + c.dt), U[s + 1], &div2[3 * s]);
+        stage_fwd<HLLC>(G, U[s], nullptr, tide(c, t), dts[s], Us[s], &div1[3 * s]);
+        stage_fwd<HLLC>(G, Us[s], &U[s], tide(c, tt[s + 1]), dts[s], U[s + 1], &div2[3 * s]);
         for (int k = 0; k < ng; ++k) {
             const int q = G.in(gi[k], gj[k]);
             const double m = (U[s + 1].h[q] + b[q]) - tgt[s * ng + k];
@@ -268,9 +312,11 @@ int run(const Cfg& c, const double* b, const double* n, const double* h0, const 
     if (!grad_b) return 0;
     State lamA{std::vector<double>(N, 0.0), std::vector<double>(N, 0.0), std::vector<double>(N, 0.0)};
     std::vector<double> gb(N, 0.0), gn(N, 0.0);
-    const SourceTerms src = src_of(c);
+    double lam_t = 0.0, carry = 0.0;   // adjoint of t_{s+1} (later steps' part); of a reused dt
     for (int s = nsteps - 1; s >= 0; --s) {
-        const double t = tt[s];
+        const double t = tt[s], dt = dts[s];
+        const SourceTerms src = src_of(c, dt);
+        double ldt = 0.0, le1 = 0.0, le2 = 0.0;
         for (int k = 0; k < ng; ++k) {
             const int q = G.in(gi[k], gj[k]);
             const double m = 2.0 * ((U[s + 1].h[q] + b[q]) - tgt[s * ng + k]);
@@ -283,28 +329,45 @@ int run(const Cfg& c, const double* b, const double* n, const double* h0, const 
         for (int k = 0; k < N; ++k) {
             const S3 dv{div2[3 * s][k], div2[3 * s + 1][k], div2[3 * s + 2][k]};
             double wh = Us[s].h[k], whu = Us[s].hu[k], whv = Us[s].hv[k];
-            body_fwd(src, c.dt, dv, n[k] * n[k], n[k] > 0.0, wh, whu, whv);
+            body_fwd(src, dt, dv, n[k] * n[k], n[k] > 0.0, wh, whu, whv);
             const double ah = 0.5 * (U[s].h[k] + wh), ahu = 0.5 * (U[s].hu[k] + whu), ahv = 0.5 * (U[s].hv[k] + whv);
             const S3 la = clamp_rev(c.g, ah, ahu, ahv, S3{lamA.h[k], lamA.hu[k], lamA.hv[k]});
             const S3 lw{0.5 * la.h, 0.5 * la.hu, 0.5 * la.hv};
             lamA.h[k] = lw.h; lamA.hu[k] = lw.hu; lamA.hv[k] = lw.hv;
-            const BodyRev r = body_rev(src, c.dt, dv, n[k], n[k] * n[k], n[k] > 0.0, Us[s].h[k], Us[s].hu[k],
+            const BodyRev r = body_rev(src, dt, dv, n[k], n[k] * n[k], n[k] > 0.0, Us[s].h[k], Us[s].hu[k],
                                        Us[s].hv[k], lw);
             lamB.h[k] = r.lv.h; lamB.hu[k] = r.lv.hu; lamB.hv[k] = r.lv.hv;
             ld[0][k] = r.ldiv.h; ld[1][k] = r.ldiv.hu; ld[2][k] = r.ldiv.hv;
             gn[k] += r.ln;
+            ldt += r.ldt;
         }
-        faces_rev<HLLC>(G, Us[s], tide(c, t + c.dt), ld, lamB, gb);
+        faces_rev<HLLC>(G, Us[s], tide(c, tt[s + 1]), ld, lamB, gb, le2);
         // stage 1
         for (int k = 0; k < N; ++k) {
             const S3 dv{div1[3 * s][k], div1[3 * s + 1][k], div1[3 * s + 2][k]};
-            const BodyRev r = body_rev(src, c.dt, dv, n[k], n[k] * n[k], n[k] > 0.0, U[s].h[k], U[s].hu[k],
+            const BodyRev r = body_rev(src, dt, dv, n[k], n[k] * n[k], n[k] > 0.0, U[s].h[k], U[s].hu[k],
                                        U[s].hv[k], S3{lamB.h[k], lamB.hu[k], lamB.hv[k]});
             lamA.h[k] += r.lv.h; lamA.hu[k] += r.lv.hu; lamA.hv[k] += r.lv.hv;
             ld[0][k] = r.ldiv.h; ld[1][k] = r.ldiv.hu; ld[2][k] = r.ldiv.hv;
             gn[k] += r.ln;
+            ldt += r.ldt;
         }
-        faces_rev<HLLC>(G, U[s], tide(c, t), ld, lamA, gb);
+        faces_rev<HLLC>(G, U[s], tide(c, t), ld, lamA, gb, le1);
+        // the clock: stage 2 ran at t_{s+1} = t_s + dt_s, stage 1 at t_s
+        const double lam_t1 = lam_t + le2 * dtide(c, tt[s + 1]);
+        const double gdt = ldt + lam_t1 + carry;
+        lam_t = lam_t1 + le1 * dtide(c, t);
+        carry = 0.0;
+        if (mode[s] == DT_CFL) {          // dt = cfl / max speed of U_s
+            const int k = amx[s];
+            const double lms = gdt * (-c.cfl / (msv[s] * msv[s]));
+            const S3 l = cfl_speed_rev(c.g, c.dx, c.dy, U[s].h[k], U[s].hu[k], U[s].hv[k], lms);
+            lamA.h[k] += l.h; lamA.hu[k] += l.hu; lamA.hv[k] += l.hv;
+        } else if (mode[s] == DT_CAPPED) { // dt = t_end - t_s
+            lam_t -= gdt;
+        } else if (mode[s] == DT_REUSED) {
+            carry = gdt;
+        }
     }
     std::memcpy(grad_b, gb.data(), N * sizeof(double));
     if (grad_n) std::memcpy(grad_n, gn.data(), N * sizeof(double));
@@ -318,11 +381,14 @@ int run(const Cfg& c, const double* b, const double* n, const double* h0, const 
 
 extern "C" {
 
-// Fixed dt from t = 0; fields are interior, x fastest (nx * ny).  With
-// grad_b == NULL only the loss is computed.
+// From t = 0 and step count 0; dt > 0 fixed, dt <= 0 the CFL dt (cfl,
+// every cfl_interval steps); every dt is bounded by t_end - t.  Fields are
+// interior, x fastest (nx * ny).  With grad_b == NULL only the loss is
+// computed.
 int adjref_run(int nx, int ny, double dx, double dy, double g, int hllc, const int* bc, double tidal_amplitude,
                double tidal_period, double coriolis_f, double inflow_velocity, double discharge_flux,
-               double clamped_elevation, double dt, const double* b, const double* n, const double* h0,
+               double clamped_elevation, double dt, double cfl, double t_end, int cfl_interval,
+               const double* b, const double* n, const double* h0,
                const double* hu0, const double* hv0, int nsteps, int ng, const int* gi, const int* gj,
                const double* tgt, double* loss, double* grad_b, double* grad_n, double* gh, double* ghu,
                double* ghv) {
@@ -332,6 +398,9 @@ int adjref_run(int nx, int ny, double dx, double dy, double g, int hllc, const i
     c.tidal_amplitude = tidal_amplitude; c.tidal_period = tidal_period; c.coriolis_f = coriolis_f;
     c.inflow_velocity = inflow_velocity; c.discharge_flux = discharge_flux; c.clamped_elevation = clamped_elevation;
     c.dt = dt;
+    c.cfl = cfl;
+    c.t_end = t_end;
+    c.cfl_interval = cfl_interval < 1 ? 1 : cfl_interval;
     if (hllc)
         return run<true>(c, b, n, h0, hu0, hv0, nsteps, ng, gi, gj, tgt, loss, grad_b, grad_n, gh, ghu, ghv);
     return run<false>(c, b, n, h0, hu0, hv0, nsteps, ng, gi, gj, tgt, loss, grad_b, grad_n, gh, ghu, ghv);

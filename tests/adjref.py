@@ -21,7 +21,9 @@ def build(outdir: Path) -> ctypes.CDLL:
 
 
 def run(lib, cfg: dict, b, n, h0, hu0, hv0, nsteps, gauges, tgt, grads=True):
-    """Returns loss, or (loss, dict of gradients b, n, h, hu, hv)."""
+    """Returns loss, or (loss, dict of gradients b, n, h, hu, hv).  cfg["dt"]
+    > 0 is a fixed dt, <= 0 the CFL dt (cfg "cfl", "cfl_interval"); dt is
+    bounded by cfg "t_end" - t."""
     nx, ny = b.shape
     f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)   # x fastest
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
@@ -37,7 +39,8 @@ def run(lib, cfg: dict, b, n, h0, hu0, hv0, nsteps, gauges, tgt, grads=True):
                         ctypes.c_int(int(cfg["hllc"])), bc, D(cfg.get("tidal_amplitude", 0.0)),
                         D(cfg.get("tidal_period", 44712.0)), D(cfg.get("coriolis_f", 0.0)),
                         D(cfg.get("inflow_velocity", 0.0)), D(cfg.get("discharge_flux", 0.0)),
-                        D(cfg.get("clamped_elevation", 0.0)), D(cfg["dt"]), *[P(a) for a in arrs],
+                        D(cfg.get("clamped_elevation", 0.0)), D(cfg["dt"]), D(cfg.get("cfl", 0.45)),
+                        D(cfg.get("t_end", 1.0e30)), ctypes.c_int(cfg.get("cfl_interval", 1)), *[P(a) for a in arrs],
                         ctypes.c_int(nsteps), ctypes.c_int(len(gi)), P(gi), P(gj), P(tg), ctypes.byref(loss),
                         *([P(o) for o in outs] if grads else [None] * 5))
     assert rc == 0

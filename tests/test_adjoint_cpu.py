@@ -24,20 +24,30 @@ def test_adjoint_building_blocks_match_finite_differences(tmp_path):
     assert res.returncode == 0 and "ALL OK" in res.stdout
 
 
-@pytest.mark.parametrize("steps, bc, coriolis, tol", [
-    (1, ("tidal", "wall", "open", "wall"), 0.0, 1e-8),
-    (6, ("wall", "wall", "wall", "wall"), 1e-3, 1e-6),
-    (6, ("inflow", "discharge", "clamped", "open"), 0.0, 1e-6),
-    (20, ("tidal", "wall", "open", "wall"), 1e-3, 1e-4),
+@pytest.mark.parametrize("steps, bc, coriolis, tol, dtcfg", [
+    (1, ("tidal", "wall", "open", "wall"), 0.0, 1e-8, {}),
+    (6, ("wall", "wall", "wall", "wall"), 1e-3, 1e-6, {}),
+    (6, ("inflow", "discharge", "clamped", "open"), 0.0, 1e-6, {}),
+    (20, ("tidal", "wall", "open", "wall"), 1e-3, 1e-4, {}),
+    # CFL dt every step / every third step, a t_end cap, a fixed dt that hits t_end
+    (6, ("tidal", "wall", "open", "wall"), 1e-3, 1e-6, dict(dt=0.0, eps=1e-7)),
+    (6, ("tidal", "wall", "open", "wall"), 1e-3, 1e-6, dict(dt=0.0, cfl_interval=3, eps=1e-7)),
+    (6, ("tidal", "wall", "open", "wall"), 1e-3, 1e-6, dict(dt=0.0, t_end=4.0, eps=1e-7)),
+    (6, ("tidal", "wall", "open", "wall"), 0.0, 1e-6, dict(dt=0.4, t_end=1.9, eps=1e-7)),
+    (20, ("tidal", "wall", "open", "wall"), 1e-3, 1e-6, dict(dt=0.0, eps=1e-7)),
 ])
-def test_reference_adjoint_matches_finite_differences(tmp_path, steps, bc, coriolis, tol):
+def test_reference_adjoint_matches_finite_differences(tmp_path, steps, bc, coriolis, tol, dtcfg):
     """The host reference adjoint against central differences of its own
     forward run, along random directions in (b, n, h0, hu0, hv0), from a
     generic state (rippled surface, moving water, bed without ties; a lake
     at rest or equal neighbouring beds sit on the kinks of minmod and of
     max(b_L, b_R), where central differences measure no gradient).  Longer
     runs cross occasional kinks (fronts, slope sign changes), hence the
-    looser tolerance there."""
+    looser tolerance there.  With the CFL dt the loss also depends on every
+    dt (update, friction, Coriolis angle, the tide's clock) and dt on the
+    cell of maximum speed; these cases use a step of 1e-7, below the spacing
+    of the kinks (without the dt terms they differ from the central
+    differences by 1e-3 to 1e-2)."""
     import numpy as np
     import adjref
     lib = adjref.build(tmp_path)
@@ -45,6 +55,9 @@ def test_reference_adjoint_matches_finite_differences(tmp_path, steps, bc, corio
     nx, ny = 14, 10
     cfg = dict(dx=10.0, dy=10.0, hllc=True, bc=bc, tidal_amplitude=0.4, tidal_period=60.0, dt=0.4,
                coriolis_f=coriolis, inflow_velocity=0.3, discharge_flux=0.5, clamped_elevation=0.0)
+    dtcfg = dict(dtcfg)
+    eps = dtcfg.pop("eps", 1e-6)
+    cfg.update(dtcfg)
     x = np.arange(nx)[:, None]
     y = np.arange(ny)[None, :]
     b = -1.0 + 1.3 * np.exp(-((x - 9) ** 2 + (y - 5) ** 2) / 6.0) + 0.01 * rng.normal(size=(nx, ny))
@@ -63,7 +76,6 @@ def test_reference_adjoint_matches_finite_differences(tmp_path, steps, bc, corio
         d["n"] *= 0.005
         for k in ("h", "hu", "hv"):
             d[k] *= wet
-        eps = 1e-6
         vals = []
         for s in (1.0, -1.0):
             vals.append(adjref.run(lib, cfg, b + s * eps * d["b"], n + s * eps * d["n"], h0 + s * eps * d["h"],

@@ -44,9 +44,11 @@ CFG = dict(dx=10.0, dy=10.0, tidal_amplitude=0.4, tidal_period=60.0, dt=0.4, inf
 
 
 def gpu_grid(nx, ny, cfg, hllc, bc, coriolis):
+    """cfg["dt"] > 0: fixed dt; 0: the CFL dt (cfg "cfl_interval"); dt bounded by cfg "t_end" - t."""
     from rakali_amd import Grid
     return Grid(nx, ny, cfg["dx"], cfg["dy"], cfl=0.45, manning_n=0.02, use_hllc=hllc, bc=bc,
-                tidal_amplitude=cfg["tidal_amplitude"], tidal_period=cfg["tidal_period"], t_end=1.0e9,
+                tidal_amplitude=cfg["tidal_amplitude"], tidal_period=cfg["tidal_period"],
+                t_end=cfg.get("t_end", 1.0e30), cfl_interval=cfg.get("cfl_interval", 1),
                 dt_fixed=cfg["dt"], coriolis_f=coriolis, inflow_velocity=cfg["inflow_velocity"],
                 discharge_flux=cfg["discharge_flux"], clamped_elevation=cfg["clamped_elevation"])
 
@@ -61,28 +63,38 @@ def load(g, b, n, h, hu, hv):
     g.set_clock(0.0, 0)
 
 
-@pytest.mark.parametrize("nx, ny, bc, hllc, coriolis, K, ty", [
-    (130, 70, ("tidal", "wall", "open", "wall"), True, 0.0, 4, 32),
-    (61, 33, ("wall", "wall", "wall", "wall"), False, 1e-3, 3, 32),
-    (121, 65, ("inflow", "discharge", "clamped", "open"), True, 0.0, 5, 16),
-    (64, 40, ("open", "tidal", "wall", "clamped"), True, 5e-4, 10, 8),
+CFL_DT = dict(dt=0.0)
+
+
+@pytest.mark.parametrize("nx, ny, bc, hllc, coriolis, K, ty, dtcfg", [
+    (130, 70, ("tidal", "wall", "open", "wall"), True, 0.0, 4, 32, {}),
+    (61, 33, ("wall", "wall", "wall", "wall"), False, 1e-3, 3, 32, {}),
+    (121, 65, ("inflow", "discharge", "clamped", "open"), True, 0.0, 5, 16, {}),
+    (64, 40, ("open", "tidal", "wall", "clamped"), True, 5e-4, 10, 8, {}),
+    # the CFL dt: every step, every third step, and bounded by t_end
+    (130, 70, ("tidal", "wall", "open", "wall"), True, 1e-3, 4, 32, CFL_DT),
+    (97, 66, ("wall", "open", "tidal", "tidal"), True, 0.0, 3, 16, dict(dt=0.0, cfl_interval=3)),
+    (70, 45, ("tidal", "clamped", "wall", "open"), False, 5e-4, 10, 8, dict(dt=0.0, t_end=3.0)),
+    (70, 45, ("tidal", "wall", "wall", "wall"), True, 0.0, 5, 32, dict(dt=0.4, t_end=2.2)),
+    # two-cell last tiles (121 = 2 x 60 - 1 + 2 columns; 50 = 2 x 24 + 2 rows) at tidal E and N edges
+    (121, 50, ("wall", "tidal", "wall", "tidal"), False, 1e-3, 4, 24, CFL_DT),
 ])
-def test_gpu_adjoint_matches_reference_cell_by_cell(ref, nx, ny, bc, hllc, coriolis, K, ty):
+def test_gpu_adjoint_matches_reference_cell_by_cell(ref, nx, ny, bc, hllc, coriolis, K, ty, dtcfg):
     import adjref
     from rakali_amd.adjoint import GaugeMisfit
     steps = 10
     rng, b, n, h, hu, hv = generic_state(nx, ny, nx + ny)
     gauges = [(int(rng.integers(0, nx)), int(rng.integers(0, ny))) for _ in range(12)]
     tgt = rng.normal(0.0, 0.05, (steps, len(gauges)))
-    cfg = dict(CFG, bc=bc, hllc=hllc, coriolis_f=coriolis)
+    cfg = dict(CFG, bc=bc, hllc=hllc, coriolis_f=coriolis, **dtcfg)
     loss_ref, want = adjref.run(ref, cfg, b, n, h, hu, hv, steps, gauges, tgt)
-    g = gpu_grid(nx, ny, CFG, hllc, bc, coriolis)
+    g = gpu_grid(nx, ny, cfg, hllc, bc, coriolis)
     load(g, b, n, h, hu, hv)
     sess = GaugeMisfit(g, steps, K, gauges, tgt)
     loss = sess.forward()
     sess.backward(tile_rows=ty)
-    print(f"[adjoint vs reference] {nx}x{ny} bc={bc} hllc={hllc} f={coriolis} loss {loss:.12e} ref {loss_ref:.12e}",
-          flush=True)
+    print(f"[adjoint vs reference] {nx}x{ny} bc={bc} hllc={hllc} f={coriolis} {dtcfg} loss {loss:.12e} "
+          f"ref {loss_ref:.12e} t={g.t}", flush=True)
     assert abs(loss - loss_ref) <= 1e-12 * abs(loss_ref)
     for k in ("b", "n", "h", "hu", "hv"):
         got = sess.gradient(k)
@@ -95,18 +107,20 @@ def test_gpu_adjoint_matches_reference_cell_by_cell(ref, nx, ny, bc, hllc, corio
     g.close()
 
 
-@pytest.mark.parametrize("coriolis", [0.0, 1e-3])
-def test_directional_derivatives_against_gpu_forward(coriolis):
-    """Central differences with step 1e-7 per cell.  The noisy start state
-    puts minmod and wave-speed ties within about 1e-6 of many cells, so a
-    step of 1e-6 crosses kinks and the difference quotient itself is off by
-    1e-3 (on this case the host reference forward shows the same); at 1e-7
-    it agrees with the adjoint to about 1e-8."""
+@pytest.mark.parametrize("coriolis, dt", [(0.0, 0.4), (1e-3, 0.4), (1e-3, 0.0)])
+def test_directional_derivatives_against_gpu_forward(coriolis, dt):
+    """Central differences with step 1e-7 per cell (1e-8 with the CFL dt).
+    The noisy start state puts minmod and wave-speed ties within about 1e-6
+    of many cells, so a step of 1e-6 crosses kinks and the difference
+    quotient itself is off by 1e-3 (on this case the host reference forward
+    shows the same); at 1e-7 it agrees with the adjoint to about 1e-8.  With
+    the CFL dt the max speed over the cells adds kinks, and 1e-7 still
+    crosses some (the host reference shows 1e-3 there too)."""
     from rakali_amd.adjoint import GaugeMisfit
     nx, ny, steps = 70, 45, 12
     rng, b, n, h, hu, hv = generic_state(nx, ny, 5)
     bc = ("tidal", "wall", "open", "wall")
-    g = gpu_grid(nx, ny, CFG, True, bc, coriolis)
+    g = gpu_grid(nx, ny, dict(CFG, dt=dt), True, bc, coriolis)
     gauges = [(int(rng.integers(0, nx)), int(rng.integers(0, ny))) for _ in range(16)]
     sess = GaugeMisfit(g, steps, 5, gauges, rng.normal(0.0, 0.05, (steps, 16)))
     load(g, b, n, h, hu, hv)
@@ -119,7 +133,7 @@ def test_directional_derivatives_against_gpu_forward(coriolis):
         d["n"] *= 0.005
         for k in ("h", "hu", "hv"):
             d[k] *= h > 0.05
-        eps = 1e-7
+        eps = 1e-7 if dt > 0.0 else 1e-8
         vals = []
         for s in (1.0, -1.0):
             load(g, b + s * eps * d["b"], n + s * eps * d["n"], h + s * eps * d["h"], hu + s * eps * d["hu"],
@@ -129,7 +143,7 @@ def test_directional_derivatives_against_gpu_forward(coriolis):
         ad = sum(float(np.sum(gr[k] * d[k])) for k in gr)
         print(f"  fd {fd:.12e}  adjoint {ad:.12e}", flush=True)
         worst = max(worst, abs(fd - ad) / abs(fd))
-    print(f"[adjoint directional] f={coriolis} loss={loss:.6e} worst rel {worst:.2e}", flush=True)
+    print(f"[adjoint directional] f={coriolis} dt={dt} loss={loss:.6e} worst rel {worst:.2e}", flush=True)
     assert worst < 1e-6
     sess.close()
     g.close()
