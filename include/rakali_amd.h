@@ -146,7 +146,9 @@ int rkg_total_mass(rkg_grid *g, double *mass);
 /* Waits for all queued work of this grid. */
 int rkg_sync(rkg_grid *g);
 
-/* Rows per tile segment of the stage kernels (1..64; default 16). */
+/* Rows per tile segment of the stage kernels (1..64; 0, the default,
+ * chooses 16 for grids of at least 8192 such tiles and 8 below).  The get
+ * call returns the rows in use. */
 int rkg_set_tile_rows(rkg_grid *g, int rows);
 int rkg_get_tile_rows(rkg_grid *g, int *rows);
 /* Keeps U^n of every step in three extra planes (24 B per cell per step of
@@ -255,7 +257,9 @@ int rkg_adjoint_set_targets(rkg_adjoint *a, const double *targets);
 int rkg_adjoint_forward(rkg_adjoint *a, double *loss);
 /* gauge eta of every step of the last forward run (n_steps x n_gauges) */
 int rkg_adjoint_gauge_eta(rkg_adjoint *a, double *eta);
-/* gradients of loss_scale * L; tile_rows of the reverse face kernel (0 = 32) */
+/* gradients of loss_scale * L; tile_rows of the reverse y-face kernel (0:
+ * chosen from the grid size so that about 1900 tiles run at once, 4 to 32
+ * rows) */
 int rkg_adjoint_backward(rkg_adjoint *a, double loss_scale, int tile_rows);
 /* adds the bed part of a still-water initial state h = max(eta0 - b, 0) */
 int rkg_adjoint_fold_still_water(rkg_adjoint *a, double eta0);

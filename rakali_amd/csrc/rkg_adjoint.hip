@@ -12,13 +12,15 @@
 //   k_adj_body  -- cell-local: RK2 average, clamp, KP07, Froude backstop,
 //                  Manning and Coriolis transposed; emits the adjoint of the
 //                  flux divergence of every cell and the Manning gradient;
-//   k_adj_faces -- tile kernel: every face and slope of the stage, with the
-//                  boundary ghosts, transposed onto the stage input and the
-//                  bed.
+//   k_adj_y     -- tile kernel, rows marched: every y face and y slope of
+//                  the stage, with the south / north ghost rows;
+//   k_adj_x     -- row-local: every x face and x slope, with the west /
+//                  east ghost columns; both add onto the adjoint of the
+//                  stage input and the bed.
 //   k_dt_rev    -- one workgroup per step: the dependence of the loss on the
 //                  step's dt (update, friction, Coriolis angle, the tide's
 //                  clock t_{s+1} = t_s + dt_s) and, for a CFL dt, on the
-//                  cell that set the max speed (k_argmax_part finds it).
+//                  cell that set the max speed (k_adj_body<1> finds it).
 // The gradient is that of the discrete scheme with its dt rule: fixed dt,
 // the CFL dt every cfl_interval steps, and the t_end bound.
 #include <hip/hip_runtime.h>
@@ -38,7 +40,7 @@ namespace {
 
 constexpr unsigned ADJ_MAGIC = 0x72616B41u;  // 'rakA'
 constexpr unsigned ADJ_DEAD = 0xDEADBEEFu;
-constexpr int AW = 60;                        // own columns per tile of k_adj_faces
+constexpr int AW = 60;                        // own columns per workgroup of k_adj_x
 
 // Own range [lo, hi) of block `s` when [0, n) is cut into blocks of `w`,
 // the last block holding at least two cells (a boundary ghost maps onto
@@ -65,9 +67,12 @@ struct AdjStage {
     double* grad_n;
     const double* hist;         // (t, dt, dt recomputed) of the step
     double* part_dt;            // per body workgroup: sum of d L / d dt of its cells
+    double* pval;               // stage 1 with the CFL dt: per body workgroup, max CFL speed of
+    long long* pidx;            // the stage input and its first cell (x fastest), else null
     double* edge_lg;            // adjoint (h, normal momentum) of every tidal ghost cell, see edge_slot
     int edge_len;               // max(nx, ny)
-    int ty;                     // rows per tile of k_adj_faces
+    int ty;                     // rows per tile of k_adj_y
+    int xrows;                  // rows per workgroup of k_adj_x
 };
 
 // Slot of ghost layer kk (0 = next to the edge) at position p along edge e
@@ -77,7 +82,6 @@ __host__ __device__ __forceinline__ long long edge_slot(int e, int kk, int p, in
 }
 
 constexpr int HIST = 3;         // doubles per step in hist
-constexpr int ARGMAX_BLOCKS = 1024;
 
 __device__ __forceinline__ SourceTerms adj_src(const GridDesc& d, double dt) {
     SourceTerms s;
@@ -119,10 +123,16 @@ __global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
     const long long P = d.plane;
     const long long n = (long long)d.nx * d.ny;
     double ldt = 0.0;
+    double best = 0.0;   // stage 1: max CFL speed of U_s and its first cell
+    long long bi = -1;
     for (long long idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
         const int j = (int)(idx / d.nx), i = (int)(idx % d.nx);
         const long long k = (long long)j * d.pitch + i;
         const double v0 = d.ah[k], v1 = d.ahu[k], v2 = d.ahv[k];
+        if (STAGE == 1 && a.pval) {
+            const double sp = adj::cfl_speed_fwd(d.g, d.dx, d.dy, v0, v1, v2);
+            if (sp > best) { best = sp; bi = idx; }
+        }
         const adj::S3 div{a.div[k], a.div[P + k], a.div[2 * P + k]};
         const double nc = d.nfield ? d.nfield[k] : d.manning_n;
         const double nsq = d.nfield ? nc * nc : src.n_sq;
@@ -152,6 +162,25 @@ __global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
     __shared__ double red4[4];
     const double tot = block_sum256(ldt, red4);
     if (threadIdx.x == 0) a.part_dt[blockIdx.x] = tot;
+    if (STAGE == 1 && a.pval) {
+        __shared__ double sv[256];
+        __shared__ long long si[256];
+        sv[threadIdx.x] = best;
+        si[threadIdx.x] = bi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                const double v = sv[threadIdx.x + s];
+                const long long x = si[threadIdx.x + s];
+                if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && x >= 0 && (si[threadIdx.x] < 0 || x < si[threadIdx.x]))) {
+                    sv[threadIdx.x] = v;
+                    si[threadIdx.x] = x;
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) { a.pval[blockIdx.x] = sv[0]; a.pidx[blockIdx.x] = si[0]; }
+    }
 }
 
 struct Acc {   // adjoint accumulator of one cell: (h, hu, hv, b)
@@ -212,31 +241,30 @@ __device__ __forceinline__ void ghost_y_rev(const GridDesc& d, double eta, int x
 
 constexpr int NWIN = 6;   // accumulator rows r-3 .. r+2
 
-// Reverse of all faces and slopes of one stage onto the stage input and
-// the bed.  One wavefront per tile; lane l holds column c0 - 2 + l, the
-// tile owns columns [c0, c1) (at most 60) and rows [j0, j1).  The rows are
+// Reverse of the y faces and y slopes of one stage (and the south / north
+// ghost rows) onto the stage input and the bed.  One wavefront per tile of
+// 64 whole columns (lane l holds column c0 + l; this direction is
+// column-local, so no halo columns) and rows [j0, j1).  The rows are
 // marched from j0 - 2 to j1: iteration r reverses the y face between rows
-// r and r+1, the y slope of row r and, for an owned row, its x faces and
-// x slopes.  Row r-1 is complete after iteration r; rows are written two
-// iterations later so that ghost rows can still be folded into them.
-// TX / TY: a west or east / a south or north edge is tidal, and the
-// adjoints of its ghost cells are kept in edge_lg for k_eta_rev.
-template <bool HLLC, int STAGE, bool TX, bool TY>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_faces(AdjStage a) {
+// r and r+1 and the y slope of row r.  Row r-1 is complete after iteration
+// r; rows are written two iterations later so that ghost rows can still be
+// folded into their source rows.  TY: a south or north edge is tidal, and
+// the adjoints of its ghost cells are kept in edge_lg for k_dt_rev.
+template <bool HLLC, int STAGE, bool TY>
+__global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
     const GridDesc& d = a.d;
     const double tau = STAGE == 1 ? a.hist[0] : a.hist[0] + a.hist[1];   // bc%time of the stage
     const int lane = threadIdx.x;
-    const int nstr = (d.nx + AW - 1) / AW;
+    const int nstr = (d.nx + WAVE - 1) / WAVE;
     const int sx = blockIdx.x % nstr, sy = blockIdx.x / nstr;
-    int c0, c1, j0, j1;
-    block_range(d.nx, AW, sx, c0, c1);
+    const int c0 = sx * WAVE;
+    int j0, j1;
     block_range(d.ny, a.ty, sy, j0, j1);
-    const int x = c0 - 2 + lane;
-    const bool xin = (unsigned)x < (unsigned)d.nx;
-    const bool own_col = x >= c0 && x < c1;
+    const bool xin = c0 + lane < d.nx;
+    const int x = xin ? c0 + lane : d.nx - 1;   // lanes past the edge repeat the last column, unwritten
     const double eta = tidal_eta(d, tau);
     const double g = d.g;
-    const double rdx = 1.0 / d.dx, rdy = 1.0 / d.dy;
+    const double rdy = 1.0 / d.dy;
     const long long P = d.plane;
 
     Acc acc[NWIN];
@@ -249,69 +277,39 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) vo
     adj::S3 llo_r{0.0, 0.0, 0.0};        // adjoint of the y lo-state of row r (from face r - 1/2)
     adj::S3 ld_r = load_ldiv(a, x, rs);
 
-    // Folds the completed row `rho` (window slot `q`) into its targets:
-    // ghost rows onto their source rows, ghost columns onto their source
-    // columns, owned cells into the adjoint arrays.
+    // Folds the completed row `rho` (window slot `q`): a ghost row onto its
+    // source rows, an owned row into the adjoint arrays.
     auto flush = [&](int rho, int q, int r) {
         if (rho < -2 || rho > d.ny + 1) return;
         const adj::C4 lg = c4(acc[q]);
         if (rho < 0 || rho >= d.ny) {
-            if (xin) {
-                int mrow, irow;
-                adj::C4 lm, li;
-                ghost_y_rev(d, eta, x, rho, lg, mrow, irow, lm, li);
-                // a tidal ghost row cell is kept once, by the tile that owns its
-                // column and the edge row (a tile next to a short last tile also
-                // reaches the ghost rows, with partial sums)
-                const int e = rho < 0 ? 2 : 3;
-                if (TY && own_col && (rho < 0 ? j0 == 0 : j1 == d.ny) && d.bc[e] == BC_TIDAL) {
-                    const long long o = edge_slot(e, rho < 0 ? -rho - 1 : rho - d.ny, x, a.edge_len);
-                    a.edge_lg[o] = lg.h;
-                    a.edge_lg[o + 1] = lg.hv;
-                }
-                // source rows are in the window: slot of row t is t - (r - 3)
-                const int qm = mrow - (r - 3), qi = irow - (r - 3);
+            int mrow, irow;
+            adj::C4 lm, li;
+            ghost_y_rev(d, eta, x, rho, lg, mrow, irow, lm, li);
+            // source rows are in the window: slot of row t is t - (r - 3)
+            const int qm = mrow - (r - 3), qi = irow - (r - 3);
 #pragma unroll
-                for (int s = 0; s < NWIN; ++s) {
-                    if (s == qm) add(acc[s], lm);
-                    if (s == qi) add(acc[s], li);
-                }
+            for (int s = 0; s < NWIN; ++s) {
+                if (s == qm) add(acc[s], lm);
+                if (s == qi) add(acc[s], li);
+            }
+            // a tidal ghost row cell is kept once, by the tile holding the edge
+            // row (a tile next to a short last tile also reaches the ghost
+            // rows, with partial sums)
+            const int e = rho < 0 ? 2 : 3;
+            if (TY && xin && (rho < 0 ? j0 == 0 : j1 == d.ny) && d.bc[e] == BC_TIDAL) {
+                const long long o = edge_slot(e, rho < 0 ? -rho - 1 : rho - d.ny, x, a.edge_len);
+                a.edge_lg[o] = lg.h;
+                a.edge_lg[o + 1] = lg.hv;
             }
             return;
         }
-        // ghost columns of an owned row fold onto their source columns
-        // (tiles whose lanes reach past an edge only)
-        adj::C4 mine = lg;
-        if (c0 - 2 < 0 || c0 + WAVE - 3 >= d.nx) {
-            adj::C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
-            int mcol = -100, icol = -100;
-            if (x == -1 || x == -2 || x == d.nx || x == d.nx + 1) {
-                ghost_x_rev(d, eta, x, rho, lg, mcol, icol, lm, li);
-                const int e = x < 0 ? 0 : 1;
-                if (TX && rho >= j0 && rho < j1 && (x < 0 ? c0 == 0 : c1 == d.nx) && d.bc[e] == BC_TIDAL) {
-                    const long long o = edge_slot(e, x < 0 ? -x - 1 : x - d.nx, rho, a.edge_len);
-                    a.edge_lg[o] = lg.h;
-                    a.edge_lg[o + 1] = lg.hu;
-                }
-            }
-            // lanes holding columns -2, -1, nx, nx + 1 (when inside the wavefront)
-            const int srcs[4] = {-c0, 1 - c0, d.nx - c0 + 2, d.nx - c0 + 3};
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int src = srcs[s];
-                if (src < 0 || src >= WAVE) continue;
-                const adj::C4 rm = sh_from(lm, src), ri = sh_from(li, src);
-                const int rmcol = __shfl(mcol, src, WAVE), ricol = __shfl(icol, src, WAVE);
-                if (x == rmcol) { mine.h += rm.h; mine.hu += rm.hu; mine.hv += rm.hv; mine.b += rm.b; }
-                if (x == ricol) { mine.h += ri.h; mine.hu += ri.hu; mine.hv += ri.hv; mine.b += ri.b; }
-            }
-        }
-        if (own_col && rho >= j0 && rho < j1) {
+        if (xin && rho >= j0 && rho < j1) {
             const long long k = (long long)rho * d.pitch + x;
-            a.lam_in[k] += mine.h;
-            a.lam_in[P + k] += mine.hu;
-            a.lam_in[2 * P + k] += mine.hv;
-            a.grad_b[k] += mine.b;
+            a.lam_in[k] += lg.h;
+            a.lam_in[P + k] += lg.hu;
+            a.lam_in[2 * P + k] += lg.hv;
+            a.grad_b[k] += lg.b;
         }
     };
 
@@ -321,9 +319,9 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) vo
         const adj::S3 ld_r1 = load_ldiv(a, x, r + 1);
         FState lo_r1, hi_r1;
         adj::states_fwd(v1, v2, v3, lo_r1, hi_r1);
-        // y face r + 1/2 (interior columns, faces -1/2 .. ny - 1/2)
+        // y face r + 1/2 (faces -1/2 .. ny - 1/2)
         adj::S3 lhi_r{0.0, 0.0, 0.0}, llo_r1{0.0, 0.0, 0.0};
-        if (xin && r >= -1 && r <= d.ny - 1) {
+        if (r >= -1 && r <= d.ny - 1) {
             const Face lf = adj::face_seed<true>(g, rdy, ld_r, ld_r1);
             double lbL = 0.0, lbR = 0.0;
             adj::face_rev<HLLC, true>(g, hi_r, v1.b, lo_r1, v2.b, lf, lhi_r, lbL, llo_r1, lbR);
@@ -338,39 +336,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) vo
             add(acc[3], lc);
             add(acc[4], lp);
         }
-        // x faces and slopes of an owned row
-        if (r >= j0 && r < j1) {
-            Cell xm, xp;
-            xm.h = sh_up(v1.h); xm.hu = sh_up(v1.hu); xm.hv = sh_up(v1.hv); xm.b = sh_up(v1.b);
-            xp.h = sh_dn(v1.h); xp.hu = sh_dn(v1.hu); xp.hv = sh_dn(v1.hv); xp.b = sh_dn(v1.b);
-            if (lane == 0) xm = vload(d, x - 1, r, eta);
-            if (lane == WAVE - 1) xp = vload(d, x + 1, r, eta);
-            FState lo_x, hi_x;
-            adj::states_fwd(xm, v1, xp, lo_x, hi_x);
-            FState R;
-            R.h = sh_dn(lo_x.h); R.hu = sh_dn(lo_x.hu); R.hv = sh_dn(lo_x.hv);
-            adj::S3 ld_next{sh_dn(ld_r.h), sh_dn(ld_r.hu), sh_dn(ld_r.hv)};
-            adj::S3 lhi_x{0.0, 0.0, 0.0}, lR{0.0, 0.0, 0.0};
-            double lbL = 0.0, lbR = 0.0;
-            if (lane < WAVE - 1 && x >= -1 && x <= d.nx - 1) {
-                const Face lf = adj::face_seed<false>(g, rdx, ld_r, ld_next);
-                adj::face_rev<HLLC, false>(g, hi_x, v1.b, R, xp.b, lf, lhi_x, lbL, lR, lbR);
-            }
-            adj::S3 llo_x{sh_up(lR.h), sh_up(lR.hu), sh_up(lR.hv)};
-            const double lb_from_left = sh_up(lbR);
-            if (lane == 0) { llo_x = adj::S3{0.0, 0.0, 0.0}; }
-            acc[3].b += lbL + (lane == 0 ? 0.0 : lb_from_left);
-            adj::C4 lm{0, 0, 0, 0}, lc{0, 0, 0, 0}, lp{0, 0, 0, 0};
-            adj::states_rev(xm, v1, xp, llo_x, lhi_x, lm, lc, lp);
-            const adj::C4 from_right = sh_dn(lm);   // lane l+1's left-neighbour part
-            const adj::C4 from_left = sh_up(lp);    // lane l-1's right-neighbour part
-            add(acc[3], lc);
-            if (lane < WAVE - 1) add(acc[3], from_right);
-            if (lane > 0) add(acc[3], from_left);
-        }
         // row r - 3 is complete and past the ghost folding window
         flush(r - 3, 0, r);
-        // shift
 #pragma unroll
         for (int q = 0; q < NWIN - 1; ++q) acc[q] = acc[q + 1];
         acc[NWIN - 1] = Acc{0.0, 0.0, 0.0, 0.0};
@@ -392,24 +359,114 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 8))) vo
     }
 }
 
-// d L / d eta_tidal of one stage from its tidal ghost cells (edge_lg), in
-// a fixed order: the eta_ext terms of bc_flather_edge (rki_boundary.F90
-// :331-343), h_g = max(eta_ext - b_g, 0) and the radiation term.
-__global__ __launch_bounds__(256) void k_eta_rev(GridDesc d, const double* hist, int stage, const double* edge_lg,
-                                                 int len, double* out) {
-    __shared__ double red[256];
-    const double eta = tidal_eta(d, stage == 1 ? hist[0] : hist[0] + hist[1]);
+// Reverse of the x faces and x slopes of one stage (and the west / east
+// ghost columns) onto the stage input and the bed.  This direction is
+// row-local: every row is independent.  One wavefront per strip of at
+// most 60 owned columns [c0, c1) and `xrows` rows; lane l holds column
+// c0 - 2 + l, so the faces and slopes that touch an owned cell are all in
+// the wavefront.  The face right of a cell is reversed by the cell's lane;
+// neighbours' parts move by lane shuffles; ghost columns fold onto their
+// source columns by lane permutes.  TX: a west or east edge is tidal (see
+// k_adj_y's TY).
+template <bool HLLC, int STAGE, bool TX>
+__global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
+    const GridDesc& d = a.d;
+    const double tau = STAGE == 1 ? a.hist[0] : a.hist[0] + a.hist[1];
+    const int lane = threadIdx.x;
+    const int nstr = (d.nx + AW - 1) / AW;
+    const int sx = blockIdx.x % nstr, sy = blockIdx.x / nstr;
+    int c0, c1;
+    block_range(d.nx, AW, sx, c0, c1);
+    const int r0 = sy * a.xrows, r1 = min(r0 + a.xrows, d.ny);
+    const int x = c0 - 2 + lane;
+    const bool own_col = x >= c0 && x < c1;
+    const bool edge_tile = c0 - 2 < 0 || c0 + WAVE - 3 >= d.nx;
+    const double eta = tidal_eta(d, tau);
+    const double g = d.g;
+    const double rdx = 1.0 / d.dx;
+    const long long P = d.plane;
+    for (int r = r0; r < r1; ++r) {
+        const Cell v1 = vload(d, x, r, eta);
+        const adj::S3 ld_r = load_ldiv(a, x, r);
+        Cell xm, xp;
+        xm.h = sh_up(v1.h); xm.hu = sh_up(v1.hu); xm.hv = sh_up(v1.hv); xm.b = sh_up(v1.b);
+        xp.h = sh_dn(v1.h); xp.hu = sh_dn(v1.hu); xp.hv = sh_dn(v1.hv); xp.b = sh_dn(v1.b);
+        if (lane == 0) xm = vload(d, x - 1, r, eta);
+        if (lane == WAVE - 1) xp = vload(d, x + 1, r, eta);
+        FState lo_x, hi_x;
+        adj::states_fwd(xm, v1, xp, lo_x, hi_x);
+        FState R;
+        R.h = sh_dn(lo_x.h); R.hu = sh_dn(lo_x.hu); R.hv = sh_dn(lo_x.hv);
+        const adj::S3 ld_next{sh_dn(ld_r.h), sh_dn(ld_r.hu), sh_dn(ld_r.hv)};
+        adj::S3 lhi_x{0.0, 0.0, 0.0}, lR{0.0, 0.0, 0.0};
+        double lbL = 0.0, lbR = 0.0;
+        if (lane < WAVE - 1 && x >= -1 && x <= d.nx - 1) {   // the face between x and x + 1
+            const Face lf = adj::face_seed<false>(g, rdx, ld_r, ld_next);
+            adj::face_rev<HLLC, false>(g, hi_x, v1.b, R, xp.b, lf, lhi_x, lbL, lR, lbR);
+        }
+        adj::S3 llo_x{sh_up(lR.h), sh_up(lR.hu), sh_up(lR.hv)};
+        const double lb_from_left = sh_up(lbR);
+        if (lane == 0) llo_x = adj::S3{0.0, 0.0, 0.0};
+        adj::C4 mine{0.0, 0.0, 0.0, lbL + (lane == 0 ? 0.0 : lb_from_left)};
+        adj::C4 lm{0, 0, 0, 0}, lc{0, 0, 0, 0}, lp{0, 0, 0, 0};
+        adj::states_rev(xm, v1, xp, llo_x, lhi_x, lm, lc, lp);
+        const adj::C4 from_right = sh_dn(lm);   // lane l+1's left-neighbour part
+        const adj::C4 from_left = sh_up(lp);    // lane l-1's right-neighbour part
+        mine.h += lc.h; mine.hu += lc.hu; mine.hv += lc.hv; mine.b += lc.b;
+        if (lane < WAVE - 1) { mine.h += from_right.h; mine.hu += from_right.hu; mine.hv += from_right.hv; mine.b += from_right.b; }
+        if (lane > 0) { mine.h += from_left.h; mine.hu += from_left.hu; mine.hv += from_left.hv; mine.b += from_left.b; }
+        // ghost columns fold onto their source columns (tiles at an edge only)
+        if (edge_tile) {
+            adj::C4 gm{0, 0, 0, 0}, gi{0, 0, 0, 0};
+            int mcol = -100, icol = -100;
+            if (x == -1 || x == -2 || x == d.nx || x == d.nx + 1) {
+                ghost_x_rev(d, eta, x, r, mine, mcol, icol, gm, gi);
+                const int e = x < 0 ? 0 : 1;
+                // kept once, by the tile holding the edge column
+                if (TX && (x < 0 ? c0 == 0 : c1 == d.nx) && d.bc[e] == BC_TIDAL) {
+                    const long long o = edge_slot(e, x < 0 ? -x - 1 : x - d.nx, r, a.edge_len);
+                    a.edge_lg[o] = mine.h;
+                    a.edge_lg[o + 1] = mine.hu;
+                }
+            }
+            // lanes holding columns -2, -1, nx, nx + 1 (when inside the wavefront)
+            const int srcs[4] = {-c0, 1 - c0, d.nx - c0 + 2, d.nx - c0 + 3};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int src = srcs[s];
+                if (src < 0 || src >= WAVE) continue;
+                const adj::C4 rm = sh_from(gm, src), ri = sh_from(gi, src);
+                const int rmcol = __shfl(mcol, src, WAVE), ricol = __shfl(icol, src, WAVE);
+                if (x == rmcol) { mine.h += rm.h; mine.hu += rm.hu; mine.hv += rm.hv; mine.b += rm.b; }
+                if (x == ricol) { mine.h += ri.h; mine.hu += ri.hu; mine.hv += ri.hv; mine.b += ri.b; }
+            }
+        }
+        if (own_col) {
+            const long long k = (long long)r * d.pitch + x;
+            a.lam_in[k] += mine.h;
+            a.lam_in[P + k] += mine.hu;
+            a.lam_in[2 * P + k] += mine.hv;
+            a.grad_b[k] += mine.b;
+        }
+    }
+}
+
+// This thread's share of d L / d eta_tidal of one stage from its tidal
+// ghost cells (edge_lg; stage input `in`, 3 planes): the eta_ext terms of
+// bc_flather_edge (rki_boundary.F90:331-343), h_g = max(eta_ext - b_g, 0)
+// and the radiation term.
+__device__ double eta_part(const GridDesc& d, double eta, const double* in, const double* edge_lg, int len) {
     double acc = 0.0;
     for (int e = 0; e < 4; ++e) {
         if (d.bc[e] != BC_TIDAL) continue;
         const bool xe = e < 2;
         const int n = xe ? d.ny : d.nx;
         const double sgn = (e == 0 || e == 2) ? -1.0 : 1.0;
-        for (int i = threadIdx.x; i < 2 * n; i += 256) {
+        for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) {
             const int kk = i / n, p = i % n;
             const int ci = xe ? (e == 0 ? 0 : d.nx - 1) : p;
             const int cj = xe ? p : (e == 2 ? 0 : d.ny - 1);
-            const Cell in0 = load_direct(d.ah, d.ahu, d.ahv, d.bed, d.pitch, ci, cj);
+            const Cell in0 = load_direct(in, in + d.plane, in + 2 * d.plane, d.bed, d.pitch, ci, cj);
             const long long o = edge_slot(e, kk, p, len);
             adj::C4 lg{edge_lg[o], 0.0, 0.0, 0.0};
             if (xe) lg.hu = edge_lg[o + 1];
@@ -421,13 +478,7 @@ __global__ __launch_bounds__(256) void k_eta_rev(GridDesc d, const double* hist,
             acc = acc + le;
         }
     }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = red[0];
+    return acc;
 }
 
 // Gauge misfit of one state: loss += sum_k (h + b - target_k)^2, eta
@@ -456,76 +507,54 @@ __global__ __launch_bounds__(256) void k_gauge_loss(const double* __restrict__ h
     if (threadIdx.x == 0) *loss = *loss + red[0];
 }
 
-// Seeds d L / d U^{n+1} at the gauges (one thread: gauges may share cells).
-__global__ void k_gauge_seed(const double* __restrict__ h, const double* __restrict__ bed, long long pitch,
-                             const int* __restrict__ gi, const int* __restrict__ gj, const double* __restrict__ tgt,
-                             int ng, double scale, double* __restrict__ lam_h, double* __restrict__ grad_b) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int k = 0; k < ng; ++k) {
-        const long long o = (long long)gj[k] * pitch + gi[k];
-        const double s = 2.0 * scale * ((h[o] + bed[o]) - tgt[k]);
-        lam_h[o] += s;
-        grad_b[o] += s;
+// Seeds d L / d U^{n+1} at the gauges.  One thread per distinct gauge cell
+// adds that cell's gauges (cell_gauges[cell_start[u] .. cell_start[u+1]))
+// in gauge order, as a single thread walking all gauges would.
+__global__ __launch_bounds__(256) void k_gauge_seed(const double* __restrict__ h, const double* __restrict__ bed,
+                                                   long long pitch, const int* __restrict__ gi,
+                                                   const int* __restrict__ gj, const double* __restrict__ tgt,
+                                                   const int* __restrict__ cell_start,
+                                                   const int* __restrict__ cell_gauges, int ncells, double scale,
+                                                   double* __restrict__ lam_h, double* __restrict__ grad_b) {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= ncells) return;
+    const int k0 = cell_gauges[cell_start[u]];
+    const long long o = (long long)gj[k0] * pitch + gi[k0];
+    const double e = h[o] + bed[o];
+    double lh = lam_h[o], gb = grad_b[o];
+    for (int q = cell_start[u]; q < cell_start[u + 1]; ++q) {
+        const int k = cell_gauges[q];
+        const double sk = 2.0 * scale * (e - tgt[k]);
+        lh += sk;
+        gb += sk;
     }
+    lam_h[o] = lh;
+    grad_b[o] = gb;
 }
 
-// (t, dt, dt recomputed) of the step that starts from control slot p
-// (the recompute test of rki_solver.F90:731).
-__global__ void k_hist(GridDesc d, const CtrlSlot* ctrl, int p, double* out) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const CtrlSlot cp = ctrl[p];
-        out[0] = cp.t;
-        out[1] = ctrl[1 - p].dt;
-        out[2] = (d.dt_fixed <= 0.0 && (cp.step_count % d.cfl_interval == 0 || cp.dt <= 0.0)) ? 1.0 : 0.0;
-    }
-}
 
-// Max CFL speed of a state and the first cell (x fastest) that attains it,
-// per workgroup; a speed of 0 has no cell (-1).
-__global__ __launch_bounds__(256) void k_argmax_part(GridDesc d, double* pval, long long* pidx) {
-    __shared__ double sv[256];
-    __shared__ long long si[256];
-    double best = 0.0;
-    long long bi = -1;
-    const long long n = (long long)d.nx * d.ny;
-    for (long long idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
-        const long long k = (idx / d.nx) * d.pitch + idx % d.nx;
-        const double v = adj::cfl_speed_fwd(d.g, d.dx, d.dy, d.ah[k], d.ahu[k], d.ahv[k]);
-        if (v > best) { best = v; bi = idx; }
-    }
-    sv[threadIdx.x] = best;
-    si[threadIdx.x] = bi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            const double v = sv[threadIdx.x + s];
-            const long long i = si[threadIdx.x + s];
-            if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && i >= 0 && (si[threadIdx.x] < 0 || i < si[threadIdx.x]))) {
-                sv[threadIdx.x] = v;
-                si[threadIdx.x] = i;
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { pval[blockIdx.x] = sv[0]; pidx[blockIdx.x] = si[0]; }
-}
 
 // Scalar part of the reverse of one step (one workgroup).  scal[0] is the
 // adjoint of t_{s+1} carried from later steps, scal[1] that of a dt that
 // the next step reused.  Adds d L / d dt * d dt / d U at the cell of
 // maximum speed to lam (the adjoint of U_s) when dt was the CFL dt.
-__global__ __launch_bounds__(256) void k_dt_rev(GridDesc d, const double* hist, const double* part_dt, int nbody,
-                                                const double* leta, const double* pval, const long long* pidx,
-                                                int nargmax, double* lam, double* scal) {
-    __shared__ double red[256];
-    __shared__ double sv[256];
-    __shared__ long long si[256];
+__global__ __launch_bounds__(1024) void k_dt_rev(GridDesc d, const double* hist, const double* part_dt, int nbody,
+                                                const double* in2, const double* edge_lg, int edge_len,
+                                                const double* pval, const long long* pidx, int nargmax, double* lam,
+                                                double* scal) {
+    __shared__ double red[3][1024];
+    __shared__ double sv[1024];
+    __shared__ long long si[1024];
+    const double t = hist[0], dt = hist[1];
     double a = 0.0;
-    for (int i = threadIdx.x; i < 2 * nbody; i += 256) a = a + part_dt[i];
-    red[threadIdx.x] = a;
+    for (int i = threadIdx.x; i < 2 * nbody; i += 1024) a = a + part_dt[i];
+    red[0][threadIdx.x] = a;
+    // d L / d eta_tidal of stage 1 (input A = d.ah, at t) and stage 2 (input B = in2, at t + dt)
+    red[1][threadIdx.x] = eta_part(d, tidal_eta(d, t), d.ah, edge_lg, edge_len);
+    red[2][threadIdx.x] = eta_part(d, tidal_eta(d, t + dt), in2, edge_lg + edge_slot(4, 0, 0, edge_len), edge_len);
     double best = 0.0;
     long long bi = -1;
-    for (int i = threadIdx.x; i < nargmax; i += 256) {
+    for (int i = threadIdx.x; i < nargmax; i += 1024) {
         const double v = pval[i];
         const long long x = pidx[i];
         if (v > best || (v == best && x >= 0 && (bi < 0 || x < bi))) { best = v; bi = x; }
@@ -533,9 +562,9 @@ __global__ __launch_bounds__(256) void k_dt_rev(GridDesc d, const double* hist, 
     sv[threadIdx.x] = best;
     si[threadIdx.x] = bi;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
-            red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + s];
             const double v = sv[threadIdx.x + s];
             const long long x = si[threadIdx.x + s];
             if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && x >= 0 && (si[threadIdx.x] < 0 || x < si[threadIdx.x]))) {
@@ -546,11 +575,10 @@ __global__ __launch_bounds__(256) void k_dt_rev(GridDesc d, const double* hist, 
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
-    const double t = hist[0], dt = hist[1];
     const bool recomputed = hist[2] != 0.0;
-    const double lam_t1 = scal[0] + leta[1] * adj::tidal_deta(d, t + dt);   // stage 2 ran at t + dt
-    const double gdt = red[0] + lam_t1 + scal[1];
-    double lam_t = lam_t1 + leta[0] * adj::tidal_deta(d, t);
+    const double lam_t1 = scal[0] + red[2][0] * adj::tidal_deta(d, t + dt);   // stage 2 ran at t + dt
+    const double gdt = red[0][0] + lam_t1 + scal[1];
+    double lam_t = lam_t1 + red[1][0] * adj::tidal_deta(d, t);
     double carry = 0.0;
     const double dt_max = d.t_end - t;
     if (d.dt_fixed > 0.0) {
@@ -596,6 +624,9 @@ struct rkg_adjoint {
     int nsteps, K, nseg, ng;
     int* gi;                    // device
     int* gj;
+    int* cell_start;            // device CSR of the distinct gauge cells (ncells + 1)
+    int* cell_gauges;           // device: gauge indices per cell, in gauge order (ng)
+    int ncells;
     double* tgt;                // device, nsteps x ng
     double* eta;                // device, nsteps x ng (gauge eta of the last forward run)
     double* ckpt;               // nseg x 3 planes
@@ -604,10 +635,10 @@ struct rkg_adjoint {
     double* ws;                 // segment workspace: A (K+1) x 3, B K x 3, div1 K x 3, div2 K x 3
     double* lam;                // lamA (3), lamB (3), lamDiv (3), grad_b, grad_n
     double* hist;               // K x HIST
-    double* part;               // part_dt (2 x body blocks), argmax values, d L / d eta_tidal (2)
+    double* part;               // part_dt (2 x body blocks), argmax values
     long long* part_idx;        // argmax cells
     double* scal;               // adjoint of the clock and of a reused dt
-    double* edge_lg;            // tidal ghost adjoints of one stage (edge_slot)
+    double* edge_lg;            // tidal ghost adjoints of stage 1, then stage 2 (edge_slot)
     int edge_len;
     double* loss_dev;
     double* loss_host;
@@ -653,30 +684,30 @@ void launch_rec(rkg_grid* g, const GridDesc& d, int p) {
         hipLaunchKernelGGL((k_stage<STAGE, false, true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, p, MODE_STEP, tlim);
 }
 
+// The stage's face reverse: k_adj_y then k_adj_x (both add into lam_in and
+// grad_b).
 template <bool HLLC, int STAGE>
-void launch_faces(rkg_grid* g, const AdjStage& st, int nblocks) {
+void launch_faces(rkg_grid* g, const AdjStage& st) {
     const int* bc = g->cfg.bc;
     const bool tx = bc[0] == BC_TIDAL || bc[1] == BC_TIDAL, ty = bc[2] == BC_TIDAL || bc[3] == BC_TIDAL;
-    if (tx && ty)
-        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, true, true>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
-    else if (tx)
-        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, true, false>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
-    else if (ty)
-        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, false, true>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
+    const int nx = g->cfg.nx, ny = g->cfg.ny;
+    const int yblocks = ((nx + WAVE - 1) / WAVE) * ((ny + st.ty - 1) / st.ty);
+    const int xblocks = ((nx + AW - 1) / AW) * ((ny + st.xrows - 1) / st.xrows);
+    if (ty)
+        hipLaunchKernelGGL((k_adj_y<HLLC, STAGE, true>), dim3(yblocks), dim3(WAVE), 0, g->stream, st);
     else
-        hipLaunchKernelGGL((k_adj_faces<HLLC, STAGE, false, false>), dim3(nblocks), dim3(WAVE), 0, g->stream, st);
+        hipLaunchKernelGGL((k_adj_y<HLLC, STAGE, false>), dim3(yblocks), dim3(WAVE), 0, g->stream, st);
+    if (tx)
+        hipLaunchKernelGGL((k_adj_x<HLLC, STAGE, true>), dim3(xblocks), dim3(WAVE), 0, g->stream, st);
+    else
+        hipLaunchKernelGGL((k_adj_x<HLLC, STAGE, false>), dim3(xblocks), dim3(WAVE), 0, g->stream, st);
 }
 
 int body_blocks(const rkg_grid* g) {
     const long long n = (long long)g->cfg.nx * g->cfg.ny;
-    return (int)std::min<long long>((n + 255) / 256, 8192);
+    return (int)std::min<long long>((n + 255) / 256, 2048);
 }
 
-int faces_tiles(const rkg_grid* g, int ty) {
-    const int nstr = (g->cfg.nx + AW - 1) / AW;
-    const int nseg = (g->cfg.ny + ty - 1) / ty;
-    return nstr * nseg;
-}
 
 }  // namespace
 
@@ -718,6 +749,26 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
         e = hipMalloc((void**)&a->gi, ng * sizeof(int));
         if (e == hipSuccess) e = hipMalloc((void**)&a->gj, ng * sizeof(int));
         if (e == hipSuccess) e = hipMemcpy(a->gi, gauge_i, ng * sizeof(int), hipMemcpyHostToDevice);
+        // distinct gauge cells in order of first appearance, each with its gauges in order
+        std::vector<long long> keys;
+        std::vector<std::vector<int>> members;
+        for (int k = 0; k < ng; ++k) {
+            const long long key = (long long)gauge_j[k] * g->cfg.nx + gauge_i[k];
+            size_t u = 0;
+            while (u < keys.size() && keys[u] != key) ++u;
+            if (u == keys.size()) { keys.push_back(key); members.emplace_back(); }
+            members[u].push_back(k);
+        }
+        std::vector<int> start(1, 0), list;
+        for (const auto& m : members) {
+            list.insert(list.end(), m.begin(), m.end());
+            start.push_back((int)list.size());
+        }
+        a->ncells = (int)keys.size();
+        if (e == hipSuccess) e = hipMalloc((void**)&a->cell_start, start.size() * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&a->cell_gauges, list.size() * sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(a->cell_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(a->cell_gauges, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(a->gj, gauge_j, ng * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc((void**)&a->tgt, (size_t)nsteps * ng * sizeof(double));
         if (e == hipSuccess) e = hipMalloc((void**)&a->eta, (size_t)nsteps * ng * sizeof(double));
@@ -732,10 +783,10 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
     if (e == hipSuccess) e = hipMalloc((void**)&a->hist, (size_t)a->K * HIST * sizeof(double));
     a->edge_len = std::max(g->cfg.nx, g->cfg.ny);
     if (e == hipSuccess)
-        e = hipMalloc((void**)&a->part, (size_t)(2 * body_blocks(g) + ARGMAX_BLOCKS + 2) * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc((void**)&a->edge_lg, (size_t)edge_slot(4, 0, 0, a->edge_len) * sizeof(double));
-    if (e == hipSuccess) e = hipMemset(a->edge_lg, 0, (size_t)edge_slot(4, 0, 0, a->edge_len) * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc((void**)&a->part_idx, ARGMAX_BLOCKS * sizeof(long long));
+        e = hipMalloc((void**)&a->part, (size_t)(3 * body_blocks(g)) * sizeof(double));   // part_dt, argmax values
+    if (e == hipSuccess) e = hipMalloc((void**)&a->edge_lg, 2 * (size_t)edge_slot(4, 0, 0, a->edge_len) * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(a->edge_lg, 0, 2 * (size_t)edge_slot(4, 0, 0, a->edge_len) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->part_idx, (size_t)body_blocks(g) * sizeof(long long));
     if (e == hipSuccess) e = hipMalloc((void**)&a->scal, 2 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&a->loss_dev, sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void**)&a->loss_host, sizeof(double), hipHostMallocDefault);
@@ -758,7 +809,7 @@ void rkg_adjoint_destroy(rkg_adjoint** pa) {
     rkg_adjoint* a = *pa;
     if (a->magic != ADJ_MAGIC) { *pa = nullptr; return; }
     if (a->g && a->g->stream) (void)hipStreamSynchronize(a->g->stream);
-    for (void* p : {(void*)a->gi, (void*)a->gj, (void*)a->tgt, (void*)a->eta, (void*)a->ckpt, (void*)a->ckpt_ctrl,
+    for (void* p : {(void*)a->gi, (void*)a->gj, (void*)a->cell_start, (void*)a->cell_gauges, (void*)a->tgt, (void*)a->eta, (void*)a->ckpt, (void*)a->ckpt_ctrl,
                     (void*)a->ws, (void*)a->lam, (void*)a->hist, (void*)a->loss_dev, (void*)a->part,
                     (void*)a->part_idx, (void*)a->scal, (void*)a->edge_lg})
         if (p) (void)hipFree(p);
@@ -837,21 +888,25 @@ int rkg_adjoint_forward(rkg_adjoint* a, double* loss) {
 int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
     if (!valid(a)) return RKG_ERR_BAD_HANDLE;
     if (!a->have_forward) { set_error("rkg_adjoint_backward before rkg_adjoint_forward"); return RKG_ERR_NOT_READY; }
-    if (tile_rows <= 0) tile_rows = 32;
-    if (tile_rows < 4 || tile_rows > 256) { set_error("tile_rows must lie in [4, 256]"); return RKG_ERR_RANGE; }
     rkg_grid* g = a->g;
+    // rows per workgroup of k_adj_y and k_adj_x: about 1900 workgroups each
+    // (two waves on each of the 1024 SIMDs, a little under, so that they all
+    // run at once)
+    const long long ny = g->cfg.ny;
+    const long long ystrips = (g->cfg.nx + WAVE - 1) / WAVE, xstrips = (g->cfg.nx + AW - 1) / AW;
+    if (tile_rows <= 0) tile_rows = (int)std::min<long long>(32, std::max<long long>(4, (ny * ystrips + 1899) / 1900));
+    if (tile_rows < 4 || tile_rows > 256) { set_error("tile_rows must lie in [4, 256]"); return RKG_ERR_RANGE; }
+    const int xrows = (int)std::min<long long>(32, std::max<long long>(1, (ny * xstrips + 1899) / 1900));
     const size_t Pl = g->plane;
     HIP_TRY(hipEventRecord(a->ev[0], g->stream));
     HIP_TRY(hipMemsetAsync(a->lam, 0, 11 * Pl * sizeof(double), g->stream));
     HIP_TRY(hipMemsetAsync(a->scal, 0, 2 * sizeof(double), g->stream));
-    const int fblocks = faces_tiles(g, tile_rows);
     const int nb = body_blocks(g);
     double* part_dt = a->part;
     double* pval = a->part + 2 * nb;
-    double* leta = a->part + 2 * nb + ARGMAX_BLOCKS;   // stage 1, stage 2
+    const long long edge_size = edge_slot(4, 0, 0, a->edge_len);
     const bool cfl_dt = g->cfg.dt_fixed <= 0.0;
-    const long long cells = (long long)g->cfg.nx * g->cfg.ny;
-    const int nargmax = cfl_dt ? (int)std::min<long long>((cells + 255) / 256, ARGMAX_BLOCKS) : 0;
+    const int nargmax = cfl_dt ? nb : 0;   // argmax partials, one per body workgroup of stage 1
     for (int s = a->nseg - 1; s >= 0; --s) {
         const int n0 = s * a->K, n1 = std::min(a->nsteps, n0 + a->K), Ks = n1 - n0;
         // replay the segment with recording
@@ -863,11 +918,11 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
         for (int k = 0; k < Ks; ++k) {
             GridDesc d1 = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
             d1.rec = wsD1(a, k);
+            d1.hist = a->hist + HIST * k;
             launch_rec<1>(g, d1, p);
-            // after stage 1: slot p holds t of the step, slot 1 - p its dt
-            hipLaunchKernelGGL(k_hist, dim3(1), dim3(64), 0, g->stream, d1, g->ctrl, p, a->hist + HIST * k);
             GridDesc d2 = d1;
             d2.rec = wsD2(a, k);
+            d2.hist = nullptr;
             launch_rec<2>(g, d2, p);
             p ^= 1;
         }
@@ -875,9 +930,9 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
         for (int k = Ks - 1; k >= 0; --k) {
             const int n = n0 + k;
             if (a->ng > 0)
-                hipLaunchKernelGGL(k_gauge_seed, dim3(1), dim3(64), 0, g->stream, wsA(a, k + 1), g->buf + 6 * Pl,
-                                   (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng, a->ng, loss_scale,
-                                   lamA(a), gradB(a));
+                hipLaunchKernelGGL(k_gauge_seed, dim3((a->ncells + 255) / 256), dim3(256), 0, g->stream, wsA(a, k + 1),
+                                   g->buf + 6 * Pl, (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng,
+                                   a->cell_start, a->cell_gauges, a->ncells, loss_scale, lamA(a), gradB(a));
             AdjStage st;
             // stage 2: input U* (B_k), U^n = A_k
             st.d = stage_desc(a, wsB(a, k), wsB(a, k), wsA(a, k + 1));
@@ -890,14 +945,15 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
             st.grad_n = gradN(a);
             st.hist = a->hist + HIST * k;
             st.part_dt = part_dt + nb;
-            st.edge_lg = a->edge_lg;
+            st.pval = nullptr;
+            st.pidx = nullptr;
+            st.edge_lg = a->edge_lg + edge_size;
             st.edge_len = a->edge_len;
             st.ty = tile_rows;
+            st.xrows = xrows;
             hipLaunchKernelGGL(k_adj_body<2>, dim3(nb), dim3(256), 0, g->stream, st);
-            if (g->cfg.use_hllc) launch_faces<true, 2>(g, st, fblocks);
-            else launch_faces<false, 2>(g, st, fblocks);
-            hipLaunchKernelGGL(k_eta_rev, dim3(1), dim3(256), 0, g->stream, st.d, st.hist, 2, a->edge_lg, a->edge_len,
-                               leta + 1);
+            if (g->cfg.use_hllc) launch_faces<true, 2>(g, st);
+            else launch_faces<false, 2>(g, st);
             // stage 1: input U^n (A_k)
             st.d = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
             st.un = nullptr;
@@ -905,16 +961,16 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
             st.lam_in = lamA(a);
             st.lam_out = lamB(a);
             st.part_dt = part_dt;
+            st.edge_lg = a->edge_lg;
+            st.pval = cfl_dt ? pval : nullptr;
+            st.pidx = cfl_dt ? a->part_idx : nullptr;
             hipLaunchKernelGGL(k_adj_body<1>, dim3(nb), dim3(256), 0, g->stream, st);
-            if (g->cfg.use_hllc) launch_faces<true, 1>(g, st, fblocks);
-            else launch_faces<false, 1>(g, st, fblocks);
-            hipLaunchKernelGGL(k_eta_rev, dim3(1), dim3(256), 0, g->stream, st.d, st.hist, 1, a->edge_lg, a->edge_len,
-                               leta);
+            if (g->cfg.use_hllc) launch_faces<true, 1>(g, st);
+            else launch_faces<false, 1>(g, st);
             // the step's dt and clock
-            if (nargmax > 0)
-                hipLaunchKernelGGL(k_argmax_part, dim3(nargmax), dim3(256), 0, g->stream, st.d, pval, a->part_idx);
-            hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(256), 0, g->stream, st.d, st.hist, part_dt, nb, leta, pval,
-                               a->part_idx, nargmax, lamA(a), a->scal);
+            hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(1024), 0, g->stream, st.d, st.hist, part_dt, nb,
+                               (const double*)wsB(a, k), (const double*)a->edge_lg, a->edge_len, pval, a->part_idx,
+                               nargmax, lamA(a), a->scal);
         }
     }
     HIP_TRY(hipGetLastError());

@@ -257,7 +257,7 @@ int rkg_create(const rkg_config* cfg, void* stream, rkg_grid** out) {
     rkg_grid* g = new rkg_grid();
     memset(g, 0, sizeof *g);
     g->cfg = *cfg;
-    g->ty = 16;   // best of 8..64 on the 4096^2 basin (fewer idle slots in the last wave of tiles)
+    g->ty = 0;    // chosen from the grid size (tile_rows_of)
     g->scheme = RKG_TIME_RK2;
     HIP_TRY(hipGetDevice(&g->device));
     if (stream) {
@@ -433,7 +433,7 @@ int rkg_sync(rkg_grid* g) {
 
 int rkg_set_tile_rows(rkg_grid* g, int rows) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
-    if (rows < 1 || rows > WAVE) { set_error("tile rows must be in 1..64"); return RKG_ERR_RANGE; }
+    if (rows < 0 || rows > WAVE) { set_error("tile rows must be in 1..64, or 0 for the size-based choice"); return RKG_ERR_RANGE; }
     g->ty = rows;
     return RKG_OK;
 }
@@ -576,7 +576,7 @@ int rkg_get_stream(rkg_grid* g, void** stream) {
 int rkg_get_tile_rows(rkg_grid* g, int* rows) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     if (!rows) return RKG_ERR_RANGE;
-    *rows = g->ty;
+    *rows = tile_rows_of(g);
     return RKG_OK;
 }
 

@@ -37,7 +37,7 @@ struct rkg_grid {
     double* mass_buf;    // device [MASS_BLOCKS + 1]
     double* mass_host;   // pinned [1]
     int parity;
-    int ty;
+    int ty;              // rows per stage tile, 0 = chosen from the grid size
     bool speed_stale;
     bool prof;             // time each stage launch with events
     hipEvent_t ev[3];
@@ -64,6 +64,16 @@ inline double* field_ptr(rkg_grid* g, int field) {
     case RKG_FIELD_N: return g->nfield;
     default: return nullptr;
     }
+}
+
+// Rows per stage tile: the set value, or (0) 16 rows when the grid has
+// at least 8192 tiles of 16 rows (several waves of tiles per wave slot, so
+// longer tiles waste less on halos), else 8 (smaller grids run one tile
+// per slot, and shorter tiles finish sooner).
+inline int tile_rows_of(const rkg_grid* g) {
+    if (g->ty > 0) return g->ty;
+    const long long strips = (g->cfg.nx + 63) / 64;
+    return strips * ((g->cfg.ny + 15) / 16) >= 8192 ? 16 : 8;
 }
 
 inline GridDesc make_desc(rkg_grid* g) {
@@ -94,9 +104,9 @@ inline GridDesc make_desc(rkg_grid* g) {
     d.clamped_elevation = c.clamped_elevation;
     d.dt_fixed = c.dt_fixed; d.t_end = c.t_end;
     d.cfl_interval = c.cfl_interval;
-    d.ty = g->ty;
+    d.ty = tile_rows_of(g);
     d.nstrips = (c.nx + WAVE - 1) / WAVE;
-    d.nsegs = (c.ny + g->ty - 1) / g->ty;
+    d.nsegs = (c.ny + d.ty - 1) / d.ty;
     d.ctrl = g->ctrl;
     d.mpart = g->mpart;
     d.ssp_coeff = g->scheme == RKG_TIME_RK43 ? 2.0 : 1.0;  // rki_timestepper.F90:60-100

@@ -82,6 +82,11 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
             q->speed_bits = active ? 0ull : cp.speed_bits;
             q->mass_out = cp.mass_out;
             q->active = active ? 1 : 0;
+            if (REC && d.hist != nullptr) {   // the step's clock for a reverse sweep (rki_solver.F90:731)
+                d.hist[0] = cp.t;
+                d.hist[1] = dt;
+                d.hist[2] = (d.dt_fixed <= 0.0 && (cp.step_count % d.cfl_interval == 0 || cp.dt <= 0.0)) ? 1.0 : 0.0;
+            }
         }
         if (!active) return;
         if (STAGE == 1) {

@@ -42,6 +42,7 @@ struct GridDesc {
     const double* nfield;         // per-cell Manning n, or null (scalar manning_n)
     double* rec;                  // 3 planes receiving each cell's flux divergence, or null
     double* mpart;                // per-tile sums of flux_h, [stage 1 tiles | stage 2 tiles], or null
+    double* hist;                 // recording stage 1: (t, dt, dt recomputed) of the step, or null
     long long plane;              // elements per plane (pitch * ny)
     double ssp_coeff;             // stable-dt multiplier of the time scheme (rki_timestepper.F90)
     double g, cfl, manning_n, coriolis_f;
