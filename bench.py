@@ -17,7 +17,9 @@ CPU baseline (the oracle, OpenMP, on a bounded sample of the same run).
 An "ensemble" object adds the batched-vs-loop comparison on 64 uneven
 tidal basins (500 steps each); an "adjoint" object the cost of the gauge-
 misfit gradient (w.r.t. bed and per-cell Manning n) on a 1024 x 1024 basin
-over 500 steps against the forward run, and its central-difference check.
+over 500 steps against the forward run, and its central-difference check;
+a "batched_adjoint" object the gradients of 32 uneven basins (256..1024
+cells a side, 300 steps) in one batch against a loop over the members.
 """
 from __future__ import annotations
 
@@ -60,6 +62,10 @@ def parse():
     ap.add_argument("--adjoint-n", type=int, default=1024, help="adjoint basin size (0 disables)")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--adjoint-fd", type=int, default=32, help="finite-difference directions")
+    ap.add_argument("--batch-members", type=int, default=32,
+                    help="members of the batched-adjoint run (0 disables)")
+    ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--batch-fd", type=int, default=8, help="finite-difference directions per member")
     return ap.parse_args()
 
 
@@ -152,6 +158,8 @@ def main():
         out["ensemble"] = run_ensemble(args.ensemble_members, args.ensemble_steps)
     if args.adjoint_n > 0:
         out["adjoint"] = run_adjoint(args.adjoint_n, args.adjoint_steps, args.adjoint_fd)
+    if args.batch_members > 0:
+        out["batched_adjoint"] = run_batched_adjoint(args.batch_members, args.batch_steps, 30, args.batch_fd)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = run_cpu_baseline(scn, args.cpu_seconds, args.cpu_threads)
     print(json.dumps(out), flush=True)
@@ -330,6 +338,131 @@ def run_adjoint(n: int, steps: int, fd_dirs: int, checkpoint: int = 50, n_gauges
         "fd_seconds": fd_s,
         "max_abs_grad_b": float(np.abs(gb).max()),
         "max_abs_grad_n": float(np.abs(gn).max()),
+    }
+
+
+def run_batched_adjoint(n_members: int, steps: int, checkpoint: int, fd_dirs: int, n_gauges: int = 64):
+    """Gradients of 32 uneven tidal basins in one batch: nx, ny log-uniform
+    in [256, 1024], dx = 50 m, the tidal-basin recipe (own bed, tide and
+    Manning field per member), still water, `steps` steps with a checkpoint
+    every `checkpoint`; loss per member as in the adjoint workload (64
+    seeded gauges, target from the bed plus 0.5 m noise and n = 0.03).
+    Batched: every member's forward run, then every member's reverse sweep,
+    with the members' launches interleaved step by step on their own
+    streams.  Loop: one member after the other.  The gradients of the two
+    must agree bit for bit; central differences (steps 1e-6 and 1e-8) along
+    `fd_dirs` seeded random directions in (b, n) check each member.
+
+    Each bed carries 1 cm of seeded roughness.  The members' narrow bumps
+    leave most of the -30 m base exactly flat, and two neighbours with the
+    same bed level put the hydrostatic reconstruction on the kink of
+    max(b_L, b_R): along a random direction the loss then changes by
+    max(db_L, db_R), which is not linear in db, and central differences
+    measure no derivative to compare with."""
+    import numpy as np
+
+    from rakali_amd import Grid
+    from rakali_amd import scenarios as S
+    from rakali_amd.adjoint import GaugeMisfit, backward_many, forward_many, start_still_water
+
+    scns = S.tidal_ensemble(n_members, seed=5, size=(256, 1024), dx=50.0, steps=steps)
+    grids, sess, beds, nfs = [], [], [], []
+    for m, scn in enumerate(scns):
+        rng = np.random.default_rng(100 + m)
+        kw = dict(scn.grid_kw)
+        n0 = kw.pop("manning_n")
+        g = Grid(scn.nx, scn.ny, scn.dx, scn.dy, manning_n=n0, **kw)
+        gauges = [(int(rng.integers(0, scn.nx)), int(rng.integers(0, scn.ny))) for _ in range(n_gauges)]
+        sess.append(GaugeMisfit(g, steps, checkpoint, gauges))
+        bed = scn.b + 0.01 * np.random.default_rng(500 + m).normal(size=scn.b.shape)
+        start_still_water(g, bed + 0.5 * rng.normal(size=scn.b.shape), np.full(scn.b.shape, 0.03))
+        grids.append(g)
+        beds.append(bed)
+        nfs.append(np.full(scn.b.shape, n0))
+    forward_many(sess)
+    for s in sess:
+        s.set_targets(s.gauge_eta())
+
+    def start(db=None, dn=None, eps=0.0):
+        for m, g in enumerate(grids):
+            b = beds[m] if db is None else beds[m] + eps * db[m]
+            nf = nfs[m] if dn is None else nfs[m] + eps * dn[m]
+            start_still_water(g, b, nf)
+
+    cells = sum(scn.nx * scn.ny for scn in scns)
+    start()
+    forward_many(sess)                        # warm-up of both passes
+    backward_many(sess)
+    start()
+    t0 = time.perf_counter()
+    losses = forward_many(sess)
+    t1 = time.perf_counter()
+    backward_many(sess)
+    for s in sess:
+        s.fold_still_water(0.0)
+    t2 = time.perf_counter()
+    grads = [(s.gradient("b"), s.gradient("n")) for s in sess]
+    # loop: one member at a time
+    t3 = time.perf_counter()
+    same = True
+    for m, (g, s) in enumerate(zip(grids, sess)):
+        start_still_water(g, beds[m], nfs[m])
+        s.forward()
+        s.backward()
+        s.fold_still_water(0.0)
+        same = same and np.array_equal(s.gradient("b"), grads[m][0]) and np.array_equal(s.gradient("n"), grads[m][1])
+    t4 = time.perf_counter()
+    # central differences, all members per batched forward pair.  Besides
+    # |fd - ad| / |fd|, the error is also given relative to sum |g_i d_i|,
+    # the size of the terms the directional derivative sums: a direction
+    # along which they cancel makes the plain relative error measure the
+    # difference quotient's rounding (about 1e-16 L / step) instead.
+    drng = np.random.default_rng(17)
+    fd_res = {}
+    for eps in (1e-6, 1e-8):
+        errs, scaled, worst = [], [], None
+        drng = np.random.default_rng(17)
+        for _ in range(fd_dirs):
+            db = [drng.normal(size=b.shape) for b in beds]
+            dn = [drng.normal(size=b.shape) * 1e-3 for b in beds]
+            start(db, dn, eps)
+            lp = forward_many(sess)
+            start(db, dn, -eps)
+            lm = forward_many(sess)
+            for m in range(n_members):
+                fd = (lp[m] - lm[m]) / (2 * eps)
+                terms = np.concatenate([(grads[m][0] * db[m]).ravel(), (grads[m][1] * dn[m]).ravel()])
+                ad = float(np.sum(terms))
+                e = abs(fd - ad) / max(abs(fd), 1e-300)
+                errs.append(e)
+                scaled.append(abs(fd - ad) / float(np.sum(np.abs(terms))))
+                if worst is None or e > worst[0]:
+                    worst = (e, m, fd, ad, float(lp[m]))
+        fd_res[eps] = {"max_rel_err": max(errs), "median_rel_err": float(np.median(errs)),
+                       "max_err_over_sum_abs_terms": max(scaled),
+                       "worst": {"member": worst[1], "fd": worst[2], "adjoint": worst[3], "loss": worst[4]}}
+    for s in sess:
+        s.close()
+    for g in grids:
+        g.close()
+    fwd_s, rev_s, loop_s = t1 - t0, t2 - t1, t4 - t3
+    return {
+        "workload": f"{n_members} tidal basins, nx, ny log-uniform in [256, 1024], dx=50 m, beds with 1 cm "
+                    f"roughness, {steps} steps, checkpoint every {checkpoint}, {n_gauges} gauges each; gradient "
+                    f"w.r.t. each member's bed and per-cell Manning n",
+        "members": n_members,
+        "cells": cells,
+        "batched_forward_s": fwd_s,
+        "batched_reverse_s": rev_s,
+        "batched_reverse_over_forward": rev_s / fwd_s,
+        "batched_cell_steps_per_s": cells * steps / (fwd_s + rev_s),
+        "loop_forward_plus_reverse_s": loop_s,
+        "speedup_vs_loop": loop_s / (fwd_s + rev_s),
+        "identical_gradients": bool(same),
+        "fd_directions_per_member": fd_dirs,
+        "fd_step_1e-6": fd_res[1e-6],
+        "fd_step_1e-8": fd_res[1e-8],
+        "total_loss": float(np.sum(losses)),
     }
 
 

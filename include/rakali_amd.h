@@ -267,6 +267,14 @@ int rkg_adjoint_fold_still_water(rkg_adjoint *a, double eta0);
 int rkg_adjoint_gradient(rkg_adjoint *a, int field, double *dst, int ld, int on_device);
 /* HIP-event time of the last forward and backward calls (ms) */
 int rkg_adjoint_timing(rkg_adjoint *a, double *forward_ms, double *backward_ms);
+/* Batches of sessions: the forward runs / reverse sweeps of n sessions whose
+ * grids have their own streams (and share n_steps and the checkpoint
+ * interval), queued step by step across the sessions so that their kernels
+ * overlap on the GPU.  Results per session equal rkg_adjoint_forward /
+ * rkg_adjoint_backward (automatic tile rows) bit for bit.  losses may be
+ * NULL.  No counterpart in the reference (ensembles of gradients). */
+int rkg_adjoint_forward_many(int n, rkg_adjoint *const *sessions, double *losses);
+int rkg_adjoint_backward_many(int n, rkg_adjoint *const *sessions, double loss_scale);
 
 #ifdef __cplusplus
 }

@@ -118,6 +118,8 @@ SIGNATURES = {
     "rkg_adjoint_fold_still_water": (ctypes.c_int, [_P, ctypes.c_double]),
     "rkg_adjoint_gradient": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int]),
     "rkg_adjoint_timing": (ctypes.c_int, [_P, _dp, _dp]),
+    "rkg_adjoint_forward_many": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _dp]),
+    "rkg_adjoint_backward_many": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_double]),
 }
 
 

@@ -8,9 +8,11 @@ with eta = h + b read at the gauge cells after every step.  ``forward``
 runs the steps from the grid's current state, keeping a checkpoint every
 ``checkpoint_interval`` steps; ``backward`` replays each segment and
 reverses it, giving dL/db, dL/dn (per-cell Manning n) and dL/d(h, hu, hv)
-of the initial state.  The gradient is that of the discrete scheme with
-the forward run's dt and boundary times held fixed; with ``dt_fixed`` it is
-the exact gradient of the step.
+of the initial state.  The gradient is that of the discrete scheme with its
+dt rule: the CFL dt of each step (through the cell of maximum speed), the
+t_end bound, or a fixed dt, and the boundary times that follow from them.
+``forward_many`` / ``backward_many`` run several sessions (one grid and
+stream each) with their launches interleaved step by step.
 
 ``BathymetryRoughnessLoss`` exposes the same computation as a
 ``torch.autograd.Function`` of the bed and Manning fields, starting from
@@ -95,6 +97,30 @@ class GaugeMisfit:
             self.close()
         except Exception:
             pass
+
+
+def _handles(sessions):
+    arr = (ctypes.c_void_p * len(sessions))(*[s._h.value for s in sessions])
+    return ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), arr
+
+
+def forward_many(sessions) -> np.ndarray:
+    """Forward runs of several sessions (each on its own grid and stream)
+    queued step by step across them; returns their losses."""
+    L = _lib.load()
+    ptr, keep = _handles(sessions)
+    out = np.zeros(len(sessions))
+    check("rkg_adjoint_forward_many", L.rkg_adjoint_forward_many(
+        len(sessions), ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def backward_many(sessions, scale: float = 1.0) -> None:
+    """Reverse sweeps of several sessions, queued step by step across them;
+    each session's gradients equal its own backward()."""
+    L = _lib.load()
+    ptr, keep = _handles(sessions)
+    check("rkg_adjoint_backward_many", L.rkg_adjoint_backward_many(len(sessions), ptr, float(scale)))
 
 
 def start_still_water(grid, b, n, eta0: float = 0.0) -> None:

@@ -838,55 +838,79 @@ int rkg_adjoint_gauge_eta(rkg_adjoint* a, double* eta) {
     return RKG_OK;
 }
 
-// nsteps steps from the grid's current state, recording checkpoints and the
-// gauge misfit; the grid is left at the final state.
-int rkg_adjoint_forward(rkg_adjoint* a, double* loss) {
-    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+}  // extern "C"
+
+namespace {
+
+// --- the forward run and the reverse sweep, as per-step pieces that queue
+// work on the session's stream without waiting, so that several sessions on
+// several streams can be interleaved step by step ---------------------------
+
+int fwd_begin(rkg_adjoint* a) {
     rkg_grid* g = a->g;
     if (g->scheme != RKG_TIME_RK2) {
         set_error("the adjoint is that of the SSP-RK2 step; set time scheme rk2");
         return RKG_ERR_UNSUPPORTED;
     }
-    const size_t Pl = g->plane;
     HIP_TRY(hipMemsetAsync(a->loss_dev, 0, sizeof(double), g->stream));
     HIP_TRY(hipEventRecord(a->ev[0], g->stream));
-    for (int n = 0; n < a->nsteps; ++n) {
-        if (n % a->K == 0) {
-            const int s = n / a->K;
-            if (g->speed_stale) {   // make the control slot current before saving it
-                int rc = rkg_step(g, 0);
-                if (rc) return rc;
-            }
-            HIP_TRY(hipMemcpyAsync(a->ckpt + (size_t)s * 3 * Pl, g->buf, 3 * Pl * sizeof(double),
-                                   hipMemcpyDeviceToDevice, g->stream));
-            HIP_TRY(hipMemcpyAsync(a->ckpt_ctrl + 2 * s, g->ctrl, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice,
-                                   g->stream));
-            a->ckpt_parity[s] = g->parity;
+    return RKG_OK;
+}
+
+// Step n of the forward run: the checkpoint at a segment start, the step,
+// the gauge misfit.
+int fwd_step(rkg_adjoint* a, int n) {
+    rkg_grid* g = a->g;
+    const size_t Pl = g->plane;
+    if (n % a->K == 0) {
+        const int s = n / a->K;
+        if (g->speed_stale) {   // make the control slot current before saving it
+            int rc = rkg_step(g, 0);
+            if (rc) return rc;
         }
-        int rc = rkg_step(g, 1);
-        if (rc) return rc;
-        if (a->ng > 0)
-            hipLaunchKernelGGL(k_gauge_loss, dim3(1), dim3(256), 0, g->stream, g->buf, g->buf + 6 * Pl,
-                               (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng, a->ng,
-                               a->eta + (size_t)n * a->ng, a->loss_dev);
+        HIP_TRY(hipMemcpyAsync(a->ckpt + (size_t)s * 3 * Pl, g->buf, 3 * Pl * sizeof(double), hipMemcpyDeviceToDevice,
+                               g->stream));
+        HIP_TRY(hipMemcpyAsync(a->ckpt_ctrl + 2 * s, g->ctrl, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice,
+                               g->stream));
+        a->ckpt_parity[s] = g->parity;
     }
+    int rc = rkg_step(g, 1);
+    if (rc) return rc;
+    if (a->ng > 0)
+        hipLaunchKernelGGL(k_gauge_loss, dim3(1), dim3(256), 0, g->stream, g->buf, g->buf + 6 * Pl,
+                           (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng, a->ng,
+                           a->eta + (size_t)n * a->ng, a->loss_dev);
+    return RKG_OK;
+}
+
+int fwd_end(rkg_adjoint* a) {
+    rkg_grid* g = a->g;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(a->ev[1], g->stream));
     HIP_TRY(hipMemcpyAsync(a->end_ctrl, g->ctrl, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipMemcpyAsync(a->loss_host, a->loss_dev, sizeof(double), hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    return RKG_OK;
+}
+
+// After the stream has drained.
+int fwd_finish(rkg_adjoint* a, double* loss) {
     HIP_TRY(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
-    a->end_parity = g->parity;
+    a->end_parity = a->g->parity;
     a->have_forward = true;
     if (loss) *loss = *a->loss_host;
     return RKG_OK;
 }
 
-// Reverse sweep: gradients of loss_scale * L with respect to the bed, the
-// per-cell Manning n and the initial state (h, hu, hv).  The grid's state
-// is left as the forward run left it.
-int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
-    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+// Per-sweep settings of one session.
+struct BwdPlan {
+    int tile_rows, xrows, nb, nargmax;
+    bool cfl_dt;
+    long long edge_size;
+    double scale;
+    int p;                      // parity of the replay
+};
+
+int bwd_begin(rkg_adjoint* a, double loss_scale, int tile_rows, BwdPlan& pl) {
     if (!a->have_forward) { set_error("rkg_adjoint_backward before rkg_adjoint_forward"); return RKG_ERR_NOT_READY; }
     rkg_grid* g = a->g;
     // rows per workgroup of k_adj_y and k_adj_x: about 1900 workgroups each
@@ -896,91 +920,208 @@ int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
     const long long ystrips = (g->cfg.nx + WAVE - 1) / WAVE, xstrips = (g->cfg.nx + AW - 1) / AW;
     if (tile_rows <= 0) tile_rows = (int)std::min<long long>(32, std::max<long long>(4, (ny * ystrips + 1899) / 1900));
     if (tile_rows < 4 || tile_rows > 256) { set_error("tile_rows must lie in [4, 256]"); return RKG_ERR_RANGE; }
-    const int xrows = (int)std::min<long long>(32, std::max<long long>(1, (ny * xstrips + 1899) / 1900));
+    pl.tile_rows = tile_rows;
+    pl.xrows = (int)std::min<long long>(32, std::max<long long>(1, (ny * xstrips + 1899) / 1900));
+    pl.nb = body_blocks(g);
+    pl.cfl_dt = g->cfg.dt_fixed <= 0.0;
+    pl.nargmax = pl.cfl_dt ? pl.nb : 0;   // argmax partials, one per body workgroup of stage 1
+    pl.edge_size = edge_slot(4, 0, 0, a->edge_len);
+    pl.scale = loss_scale;
+    pl.p = 0;
     const size_t Pl = g->plane;
     HIP_TRY(hipEventRecord(a->ev[0], g->stream));
     HIP_TRY(hipMemsetAsync(a->lam, 0, 11 * Pl * sizeof(double), g->stream));
     HIP_TRY(hipMemsetAsync(a->scal, 0, 2 * sizeof(double), g->stream));
-    const int nb = body_blocks(g);
+    return RKG_OK;
+}
+
+int seg_steps(const rkg_adjoint* a, int s) { return std::min(a->nsteps, (s + 1) * a->K) - s * a->K; }
+
+// Start of the replay of segment s: its checkpoint into A_0 and the clock.
+int bwd_replay_begin(rkg_adjoint* a, int s, BwdPlan& pl) {
+    rkg_grid* g = a->g;
+    const size_t Pl = g->plane;
+    HIP_TRY(hipMemcpyAsync(wsA(a, 0), a->ckpt + (size_t)s * 3 * Pl, 3 * Pl * sizeof(double), hipMemcpyDeviceToDevice,
+                           g->stream));
+    HIP_TRY(hipMemcpyAsync(g->ctrl, a->ckpt_ctrl + 2 * s, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice, g->stream));
+    pl.p = a->ckpt_parity[s];
+    return RKG_OK;
+}
+
+// Step k of the segment's replay, recording U^n, U*, both divergences and
+// the clock.
+void bwd_replay_step(rkg_adjoint* a, int k, BwdPlan& pl) {
+    rkg_grid* g = a->g;
+    GridDesc d1 = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
+    d1.rec = wsD1(a, k);
+    d1.hist = a->hist + HIST * k;
+    launch_rec<1>(g, d1, pl.p);
+    GridDesc d2 = d1;
+    d2.rec = wsD2(a, k);
+    d2.hist = nullptr;
+    launch_rec<2>(g, d2, pl.p);
+    pl.p ^= 1;
+}
+
+// Reverse of step k of segment s (global step n0 + k).
+void bwd_reverse_step(rkg_adjoint* a, int s, int k, const BwdPlan& pl) {
+    rkg_grid* g = a->g;
+    const size_t Pl = g->plane;
+    const int n = s * a->K + k;
+    const int nb = pl.nb;
     double* part_dt = a->part;
     double* pval = a->part + 2 * nb;
-    const long long edge_size = edge_slot(4, 0, 0, a->edge_len);
-    const bool cfl_dt = g->cfg.dt_fixed <= 0.0;
-    const int nargmax = cfl_dt ? nb : 0;   // argmax partials, one per body workgroup of stage 1
-    for (int s = a->nseg - 1; s >= 0; --s) {
-        const int n0 = s * a->K, n1 = std::min(a->nsteps, n0 + a->K), Ks = n1 - n0;
-        // replay the segment with recording
-        HIP_TRY(hipMemcpyAsync(wsA(a, 0), a->ckpt + (size_t)s * 3 * Pl, 3 * Pl * sizeof(double),
-                               hipMemcpyDeviceToDevice, g->stream));
-        HIP_TRY(hipMemcpyAsync(g->ctrl, a->ckpt_ctrl + 2 * s, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice,
-                               g->stream));
-        int p = a->ckpt_parity[s];
-        for (int k = 0; k < Ks; ++k) {
-            GridDesc d1 = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
-            d1.rec = wsD1(a, k);
-            d1.hist = a->hist + HIST * k;
-            launch_rec<1>(g, d1, p);
-            GridDesc d2 = d1;
-            d2.rec = wsD2(a, k);
-            d2.hist = nullptr;
-            launch_rec<2>(g, d2, p);
-            p ^= 1;
-        }
-        // reverse the steps of the segment
-        for (int k = Ks - 1; k >= 0; --k) {
-            const int n = n0 + k;
-            if (a->ng > 0)
-                hipLaunchKernelGGL(k_gauge_seed, dim3((a->ncells + 255) / 256), dim3(256), 0, g->stream, wsA(a, k + 1),
-                                   g->buf + 6 * Pl, (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng,
-                                   a->cell_start, a->cell_gauges, a->ncells, loss_scale, lamA(a), gradB(a));
-            AdjStage st;
-            // stage 2: input U* (B_k), U^n = A_k
-            st.d = stage_desc(a, wsB(a, k), wsB(a, k), wsA(a, k + 1));
-            st.un = wsA(a, k);
-            st.div = wsD2(a, k);
-            st.lam_in = lamB(a);
-            st.lam_out = lamA(a);
-            st.lam_div = lamDiv(a);
-            st.grad_b = gradB(a);
-            st.grad_n = gradN(a);
-            st.hist = a->hist + HIST * k;
-            st.part_dt = part_dt + nb;
-            st.pval = nullptr;
-            st.pidx = nullptr;
-            st.edge_lg = a->edge_lg + edge_size;
-            st.edge_len = a->edge_len;
-            st.ty = tile_rows;
-            st.xrows = xrows;
-            hipLaunchKernelGGL(k_adj_body<2>, dim3(nb), dim3(256), 0, g->stream, st);
-            if (g->cfg.use_hllc) launch_faces<true, 2>(g, st);
-            else launch_faces<false, 2>(g, st);
-            // stage 1: input U^n (A_k)
-            st.d = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
-            st.un = nullptr;
-            st.div = wsD1(a, k);
-            st.lam_in = lamA(a);
-            st.lam_out = lamB(a);
-            st.part_dt = part_dt;
-            st.edge_lg = a->edge_lg;
-            st.pval = cfl_dt ? pval : nullptr;
-            st.pidx = cfl_dt ? a->part_idx : nullptr;
-            hipLaunchKernelGGL(k_adj_body<1>, dim3(nb), dim3(256), 0, g->stream, st);
-            if (g->cfg.use_hllc) launch_faces<true, 1>(g, st);
-            else launch_faces<false, 1>(g, st);
-            // the step's dt and clock
-            hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(1024), 0, g->stream, st.d, st.hist, part_dt, nb,
-                               (const double*)wsB(a, k), (const double*)a->edge_lg, a->edge_len, pval, a->part_idx,
-                               nargmax, lamA(a), a->scal);
-        }
-    }
+    if (a->ng > 0)
+        hipLaunchKernelGGL(k_gauge_seed, dim3((a->ncells + 255) / 256), dim3(256), 0, g->stream, wsA(a, k + 1),
+                           g->buf + 6 * Pl, (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng,
+                           a->cell_start, a->cell_gauges, a->ncells, pl.scale, lamA(a), gradB(a));
+    AdjStage st;
+    // stage 2: input U* (B_k), U^n = A_k
+    st.d = stage_desc(a, wsB(a, k), wsB(a, k), wsA(a, k + 1));
+    st.un = wsA(a, k);
+    st.div = wsD2(a, k);
+    st.lam_in = lamB(a);
+    st.lam_out = lamA(a);
+    st.lam_div = lamDiv(a);
+    st.grad_b = gradB(a);
+    st.grad_n = gradN(a);
+    st.hist = a->hist + HIST * k;
+    st.part_dt = part_dt + nb;
+    st.pval = nullptr;
+    st.pidx = nullptr;
+    st.edge_lg = a->edge_lg + pl.edge_size;
+    st.edge_len = a->edge_len;
+    st.ty = pl.tile_rows;
+    st.xrows = pl.xrows;
+    hipLaunchKernelGGL(k_adj_body<2>, dim3(nb), dim3(256), 0, g->stream, st);
+    if (g->cfg.use_hllc) launch_faces<true, 2>(g, st);
+    else launch_faces<false, 2>(g, st);
+    // stage 1: input U^n (A_k)
+    st.d = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
+    st.un = nullptr;
+    st.div = wsD1(a, k);
+    st.lam_in = lamA(a);
+    st.lam_out = lamB(a);
+    st.part_dt = part_dt;
+    st.edge_lg = a->edge_lg;
+    st.pval = pl.cfl_dt ? pval : nullptr;
+    st.pidx = pl.cfl_dt ? a->part_idx : nullptr;
+    hipLaunchKernelGGL(k_adj_body<1>, dim3(nb), dim3(256), 0, g->stream, st);
+    if (g->cfg.use_hllc) launch_faces<true, 1>(g, st);
+    else launch_faces<false, 1>(g, st);
+    // the step's dt and clock
+    hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(1024), 0, g->stream, st.d, st.hist, part_dt, nb,
+                       (const double*)wsB(a, k), (const double*)a->edge_lg, a->edge_len, pval, a->part_idx,
+                       pl.nargmax, lamA(a), a->scal);
+}
+
+int bwd_end(rkg_adjoint* a) {
+    rkg_grid* g = a->g;
     HIP_TRY(hipGetLastError());
     // the grid's control slots back to the end of the forward run
     HIP_TRY(hipMemcpyAsync(g->ctrl, a->end_ctrl, 2 * sizeof(CtrlSlot), hipMemcpyHostToDevice, g->stream));
     g->parity = a->end_parity;
     HIP_TRY(hipEventRecord(a->ev[1], g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    return RKG_OK;
+}
+
+int bwd_finish(rkg_adjoint* a) {
     HIP_TRY(hipEventElapsedTime(&a->bwd_ms, a->ev[0], a->ev[1]));
     return RKG_OK;
+}
+
+// Sessions of one batch: valid, distinct grids on distinct streams, with
+// one step count and checkpoint interval.
+int check_batch(int n, rkg_adjoint* const* as) {
+    if (n < 1 || !as) { set_error("need at least one session"); return RKG_ERR_RANGE; }
+    for (int m = 0; m < n; ++m) {
+        if (!valid(as[m])) return RKG_ERR_BAD_HANDLE;
+        if (as[m]->nsteps != as[0]->nsteps || as[m]->K != as[0]->K) {
+            set_error("sessions of a batch need the same step count and checkpoint interval");
+            return RKG_ERR_RANGE;
+        }
+        for (int q = 0; q < m; ++q)
+            if (as[q]->g == as[m]->g || as[q]->g->stream == as[m]->g->stream) {
+                set_error("sessions %d and %d share a grid or a stream", q, m);
+                return RKG_ERR_RANGE;
+            }
+    }
+    return RKG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// nsteps steps from the grid's current state, recording checkpoints and the
+// gauge misfit; the grid is left at the final state.
+int rkg_adjoint_forward(rkg_adjoint* a, double* loss) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    int rc = fwd_begin(a);
+    for (int n = 0; !rc && n < a->nsteps; ++n) rc = fwd_step(a, n);
+    if (!rc) rc = fwd_end(a);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(a->g->stream));
+    return fwd_finish(a, loss);
+}
+
+// Reverse sweep: gradients of loss_scale * L with respect to the bed, the
+// per-cell Manning n and the initial state (h, hu, hv).  The grid's state
+// is left as the forward run left it.
+int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
+    if (!valid(a)) return RKG_ERR_BAD_HANDLE;
+    BwdPlan pl;
+    int rc = bwd_begin(a, loss_scale, tile_rows, pl);
+    if (rc) return rc;
+    for (int s = a->nseg - 1; s >= 0; --s) {
+        rc = bwd_replay_begin(a, s, pl);
+        if (rc) return rc;
+        for (int k = 0; k < seg_steps(a, s); ++k) bwd_replay_step(a, k, pl);
+        for (int k = seg_steps(a, s) - 1; k >= 0; --k) bwd_reverse_step(a, s, k, pl);
+    }
+    rc = bwd_end(a);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(a->g->stream));
+    return bwd_finish(a);
+}
+
+// The forward runs of n sessions (one grid and stream each), queued step
+// by step across the sessions so that their kernels overlap.
+int rkg_adjoint_forward_many(int n, rkg_adjoint* const* as, double* losses) {
+    int rc = check_batch(n, as);
+    if (rc) return rc;
+    for (int m = 0; m < n && !rc; ++m) rc = fwd_begin(as[m]);
+    for (int k = 0; k < as[0]->nsteps && !rc; ++k)
+        for (int m = 0; m < n && !rc; ++m) rc = fwd_step(as[m], k);
+    for (int m = 0; m < n && !rc; ++m) rc = fwd_end(as[m]);
+    if (rc) return rc;
+    for (int m = 0; m < n; ++m) HIP_TRY(hipStreamSynchronize(as[m]->g->stream));
+    for (int m = 0; m < n && !rc; ++m) rc = fwd_finish(as[m], losses ? losses + m : nullptr);
+    return rc;
+}
+
+// The reverse sweeps of n sessions, queued step by step across the
+// sessions (replays, then reverse steps, segment by segment).  Each
+// session's gradients are those of rkg_adjoint_backward.
+int rkg_adjoint_backward_many(int n, rkg_adjoint* const* as, double loss_scale) {
+    int rc = check_batch(n, as);
+    if (rc) return rc;
+    std::vector<BwdPlan> pl(n);
+    for (int m = 0; m < n && !rc; ++m) rc = bwd_begin(as[m], loss_scale, 0, pl[m]);
+    for (int s = as[0]->nseg - 1; s >= 0 && !rc; --s) {
+        for (int m = 0; m < n && !rc; ++m) rc = bwd_replay_begin(as[m], s, pl[m]);
+        if (rc) break;
+        const int Ks = seg_steps(as[0], s);
+        for (int k = 0; k < Ks; ++k)
+            for (int m = 0; m < n; ++m) bwd_replay_step(as[m], k, pl[m]);
+        for (int k = Ks - 1; k >= 0; --k)
+            for (int m = 0; m < n; ++m) bwd_reverse_step(as[m], s, k, pl[m]);
+    }
+    for (int m = 0; m < n && !rc; ++m) rc = bwd_end(as[m]);
+    if (rc) return rc;
+    for (int m = 0; m < n; ++m) HIP_TRY(hipStreamSynchronize(as[m]->g->stream));
+    for (int m = 0; m < n && !rc; ++m) rc = bwd_finish(as[m]);
+    return rc;
 }
 
 // Adds the adjoint of a still-water initial state (h = max(eta0 - b, 0),
