@@ -77,7 +77,12 @@ int rki_set_initial_h(void *handle, const double *h, int nx, int ny);
 int rki_set_h(void *handle, const double *h, int nx, int ny);
 int rki_set_hu(void *handle, const double *hu, int nx, int ny);
 int rki_set_hv(void *handle, const double *hv, int nx, int ny);
-/* rki_api.F90:896-1034 -- forcing fields: -5 (not implemented here). */
+/* rki_api.F90:896-1034 -- surface forcing fields, interior (nx, ny).  Wind
+ * stress (Pa), atmospheric pressure (Pa) and precipitation (m/s) switch
+ * the forcing on (state%forcing%active) and enter every stage; until the
+ * wind is set it is the namelist's uniform wind_stress_x/y.  Heat flux and
+ * E-P are stored: they feed only the multilayer path, which this build
+ * does not run. */
 int rki_set_wind(void *handle, const double *wx, const double *wy, int nx, int ny);
 int rki_set_atm_pressure(void *handle, const double *p, int nx, int ny);
 int rki_set_precipitation(void *handle, const double *p, int nx, int ny);

@@ -70,6 +70,16 @@ extern "C" {
  * The reference's manning_n is a scalar (rki_barotropic_state.F90:18-110);
  * the field is an addition for gradients with respect to roughness. */
 #define RKG_FIELD_N 10
+/* Surface forcing planes (state%forcing, rki_surface_forcing.F90), allocated
+ * by rkg_set_forcing and zero until uploaded: wind stress (Pa, the x and y
+ * components), atmospheric pressure (Pa) and precipitation rate (m/s).
+ * The pressure gradient is central, with pressure 0 outside the grid (the
+ * reference's ghost cells, which its setters never write,
+ * rki_api.F90:926-980). */
+#define RKG_FIELD_WIND_X 11
+#define RKG_FIELD_WIND_Y 12
+#define RKG_FIELD_PRESSURE 13
+#define RKG_FIELD_PRECIP 14
 
 #define RKG_MAX_TIDAL_CONSTITUENTS 10
 
@@ -174,6 +184,21 @@ int rkg_get_time_scheme(rkg_grid *g, int *scheme);
  * in a fixed order (per tile, then over tiles). */
 int rkg_set_mass_budget(rkg_grid *g, int enable);
 int rkg_get_mass_budget(rkg_grid *g, double *mass_out);
+/* Surface forcing on (active != 0) or off: state%forcing%active, which the
+ * reference's setters rki_set_wind / rki_set_atm_pressure /
+ * rki_set_precipitation switch on (rki_api.F90:898-980).  While on, each
+ * stage adds the wind stress and the pressure gradient to hu, hv of wet
+ * cells and the precipitation to h of every cell (rki_kernel_fused.F90:
+ * 273-289).  The first call allocates the four RKG_FIELD_WIND_X..PRECIP
+ * planes (32 B per cell); they stay until rkg_destroy.  The adjoint does
+ * not support forcing. */
+int rkg_set_forcing(rkg_grid *g, int active);
+int rkg_get_forcing(rkg_grid *g, int *active);
+/* Surface mass source state%barotropic%mass_src (accumulate_mass_src,
+ * rki_solver.F90:539-606): while the mass budget is enabled and the
+ * forcing is on, each SSP-RK2 step adds 0.5*dt*sum(precipitation)*dx*dy
+ * per stage.  rkg_set_mass_budget(g, 1) starts it at 0. */
+int rkg_get_mass_source(rkg_grid *g, double *mass_src);
 /* Reads / replaces the scalar configuration (nx and ny cannot change):
  * boundary values, tide, friction, CFL, t_end, HLL/HLLC. */
 int rkg_get_config(rkg_grid *g, rkg_config *cfg);

@@ -51,13 +51,17 @@ class Params(ctypes.Structure):
         ("discharge_flux", ctypes.c_double),
         ("clamped_elevation", ctypes.c_double),
         ("time_scheme", ctypes.c_int),
+        ("wind_x", ctypes.c_void_p), ("wind_y", ctypes.c_void_p),
+        ("atm_pressure", ctypes.c_void_p), ("precipitation", ctypes.c_void_p),
+        ("forcing_active", ctypes.c_int),
     ]
 
 
 class Clock(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double),
                 ("mass_out", ctypes.c_double),
-                ("step_count", ctypes.c_longlong)]
+                ("step_count", ctypes.c_longlong),
+                ("mass_src", ctypes.c_double)]
 
 
 _lib = None

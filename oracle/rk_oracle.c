@@ -29,6 +29,7 @@
 #define FROUDE_BACKSTOP 50.0
 /* rki_constants.F90:114, TWO_PI = 8*atan(1) folded */
 #define TWO_PI 6.283185307179586
+#define RHO_WATER 1035.0   /* rki_constants.F90 */
 
 static int g_threads = 0;
 static int g_libm = 0; /* 0: correctly rounded pow/sin/cos (rk_crmath.h), 1: platform libm */
@@ -419,13 +420,33 @@ static inline void clamp_desing(double g, double eps4, double *h_val,
     *h_val = h; *hu_val = hu; *hv_val = hv;
 }
 
-/* Euler update + clamp + Manning + Coriolis for one cell,
- * rki_kernel_fused.F90:228-290 (forcing inactive). */
+/* Forcing at one cell: wind stress, the central pressure gradient and the
+ * precipitation rate (rki_kernel_fused.F90:268-275 and :286-289).  Off-grid
+ * pressure is the ghost value, which no setter writes (0). */
+typedef struct { int on; double wx, wy, dpdx, dpdy, rain; } orc_frc;
+
+static inline double frc_at(const double *a, size_t k) { return a ? a[k] : 0.0; }
+
+static inline orc_frc forcing_at(const orc_params *p, int nxt, int i, int j)
+{
+    orc_frc f = {0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!p->forcing_active) return f;
+    f.on = 1;
+    f.wx = frc_at(p->wind_x, IX(i, j));
+    f.wy = frc_at(p->wind_y, IX(i, j));
+    f.dpdx = (frc_at(p->atm_pressure, IX(i + 1, j)) - frc_at(p->atm_pressure, IX(i - 1, j))) / (2.0 * p->dx);
+    f.dpdy = (frc_at(p->atm_pressure, IX(i, j + 1)) - frc_at(p->atm_pressure, IX(i, j - 1))) / (2.0 * p->dy);
+    f.rain = frc_at(p->precipitation, IX(i, j));
+    return f;
+}
+
+/* Euler update + clamp + Manning + forcing + Coriolis + precipitation for
+ * one cell, rki_kernel_fused.F90:228-290. */
 static inline void stage_body(const orc_params *p, double dt, double eps4,
                               double n_sq, int has_friction, int has_coriolis,
                               double cosf, double sinf, double fh, double fhu,
                               double fhv, double *h_val, double *hu_val,
-                              double *hv_val)
+                              double *hv_val, orc_frc frc)
 {
     const double g = p->g;
     double h = *h_val + dt * fh;
@@ -442,12 +463,19 @@ static inline void stage_body(const orc_params *p, double dt, double eps4,
             hu = hu / denom;
             hv = hv / denom;
         }
+        if (frc.on) {   /* :268-275 */
+            hu = hu + dt * frc.wx / RHO_WATER;
+            hv = hv + dt * frc.wy / RHO_WATER;
+            hu = hu - dt * h * frc.dpdx / RHO_WATER;
+            hv = hv - dt * h * frc.dpdy / RHO_WATER;
+        }
         if (has_coriolis) {
             double hu_old = hu, hv_old = hv;
             hu = hu_old * cosf + hv_old * sinf;
             hv = -hu_old * sinf + hv_old * cosf;
         }
     }
+    if (frc.on) h = h + dt * frc.rain;   /* :286-289, every cell */
     *h_val = h; *hu_val = hu; *hv_val = hv;
 }
 
@@ -475,7 +503,7 @@ void orc_fused_stage1(const orc_params *p, double *h, double *hu, double *hv,
             double hh = h[k], hhu = hu[k], hhv = hv[k];
             h0[k] = hh; hu0[k] = hhu; hv0[k] = hhv;
             stage_body(p, dt, eps4, n_sq, has_friction, has_coriolis, cosf, sinf,
-                       fh[k], fhu[k], fhv[k], &hh, &hhu, &hhv);
+                       fh[k], fhu[k], fhv[k], &hh, &hhu, &hhv, forcing_at(p, nxt, i, j));
             h[k] = hh; hu[k] = hhu; hv[k] = hhv;
         }
     }
@@ -495,7 +523,7 @@ void orc_fused_stage2(const orc_params *p, double *h, double *hu, double *hv,
             size_t k = IX(i, j);
             double hh = h[k], hhu = hu[k], hhv = hv[k];
             stage_body(p, dt, eps4, n_sq, has_friction, has_coriolis, cosf, sinf,
-                       fh[k], fhu[k], fhv[k], &hh, &hhu, &hhv);
+                       fh[k], fhu[k], fhv[k], &hh, &hhu, &hhv, forcing_at(p, nxt, i, j));
             /* :435-438 RK2 average, then :440-463 final clamp */
             hh = 0.5 * (h0[k] + hh);
             hhu = 0.5 * (hu0[k] + hhu);
@@ -520,7 +548,7 @@ void orc_fused_stage_generic(const orc_params *p, double *h, double *hu, double 
             size_t k = IX(i, j);
             double hh = h[k], hhu = hu[k], hhv = hv[k];
             stage_body(p, dt, eps4, n_sq, has_friction, has_coriolis, cosf, sinf,
-                       fh[k], fhu[k], fhv[k], &hh, &hhu, &hhv);
+                       fh[k], fhu[k], fhv[k], &hh, &hhu, &hhv, forcing_at(p, nxt, i, j));
             h[k] = hh; hu[k] = hhu; hv[k] = hhv;
         }
     }
@@ -849,12 +877,16 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
         orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);
         clk->mass_out = clk->mass_out - 0.5 * dt * interior_sum(p, fh) * area;
         orc_fused_stage1(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt);
+        if (p->forcing_active && p->precipitation)   /* accumulate_mass_src :569-601 */
+            clk->mass_src = clk->mass_src + 0.5 * dt * interior_sum(p, p->precipitation) * area;
         /* stage 2 :789-814, bc%time = time + dt */
         double time2 = time + dt;
         orc_apply_bc(p, time2, h, hu, hv, b);
         orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);
         clk->mass_out = clk->mass_out - 0.5 * dt * interior_sum(p, fh) * area;
         orc_fused_stage2(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt);
+        if (p->forcing_active && p->precipitation)
+            clk->mass_src = clk->mass_src + 0.5 * dt * interior_sum(p, p->precipitation) * area;
         /* rki_api.F90:378 */
         clk->t = clk->t + dt;
         ++taken;

@@ -40,6 +40,11 @@ typedef struct {
     double tc_amp[ORC_MAX_TC], tc_phase[ORC_MAX_TC], tc_omega[ORC_MAX_TC];
     double inflow_velocity, discharge_flux, clamped_elevation;
     int time_scheme;         /* ORC_TIME_* (rki_timestepper.F90:27-36) */
+    /* Surface forcing (rki_surface_forcing.F90): ghosted arrays like the
+     * state, or NULL (zero); applied when forcing_active
+     * (rki_kernel_fused.F90:268-283, :286-289). */
+    const double *wind_x, *wind_y, *atm_pressure, *precipitation;
+    int forcing_active;
 } orc_params;
 
 /* Explicit SSP schemes, tags as rki_timestepper.F90:27-33.  ORC_TIME_RK2
@@ -53,6 +58,7 @@ typedef struct {
     double dt;           /* state%dt of the last step */
     double mass_out;     /* boundary mass budget (rki_solver.F90:507-537) */
     long long step_count;
+    double mass_src;     /* surface source budget (rki_solver.F90:539-606) */
 } orc_clock;
 
 /* Fills default parameters (rki_config.F90 defaults, rki_constants.F90). */

@@ -26,7 +26,8 @@ ERRORS = {-1: "not ready", -2: "argument out of range", -4: "HIP runtime error",
 BC = {"wall": 1, "open": 2, "tidal": 3, "inflow": 5, "discharge": 6,
       "clamped": 7, "sponge": 8}
 FIELDS = {"h": 0, "hu": 1, "hv": 2, "b": 3, "h0": 4, "hu0": 5, "hv0": 6,
-          "h_star": 7, "hu_star": 8, "hv_star": 9, "n": 10}
+          "h_star": 7, "hu_star": 8, "hv_star": 9, "n": 10,
+          "wind_x": 11, "wind_y": 12, "atm_pressure": 13, "precipitation": 14}
 
 
 class Config(ctypes.Structure):
@@ -93,6 +94,9 @@ SIGNATURES = {
     "rkg_get_time_scheme": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     "rkg_set_mass_budget": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_get_mass_budget": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "rkg_set_forcing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rkg_get_forcing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "rkg_get_mass_source": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
     "rkg_get_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_set_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_ensemble_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, _P, ctypes.POINTER(_P)]),

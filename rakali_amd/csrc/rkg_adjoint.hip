@@ -732,6 +732,10 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
         set_error("the adjoint is that of the SSP-RK2 step; set time scheme rk2");
         return RKG_ERR_UNSUPPORTED;
     }
+    if (g->forcing) {
+        set_error("the adjoint does not cover the surface forcing; switch it off");
+        return RKG_ERR_UNSUPPORTED;
+    }
     int rc = rkg_use_manning_field(g, 1);
     if (rc) return rc;
     rkg_adjoint* a = new rkg_adjoint();
@@ -850,6 +854,10 @@ int fwd_begin(rkg_adjoint* a) {
     rkg_grid* g = a->g;
     if (g->scheme != RKG_TIME_RK2) {
         set_error("the adjoint is that of the SSP-RK2 step; set time scheme rk2");
+        return RKG_ERR_UNSUPPORTED;
+    }
+    if (g->forcing) {
+        set_error("the adjoint does not cover the surface forcing; switch it off");
         return RKG_ERR_UNSUPPORTED;
     }
     HIP_TRY(hipMemsetAsync(a->loss_dev, 0, sizeof(double), g->stream));

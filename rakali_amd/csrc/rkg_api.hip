@@ -73,7 +73,7 @@ int refresh_speed(rkg_grid* g) {
     HIP_TRY(hipStreamSynchronize(g->stream));
     const CtrlSlot c = g->ctrl_host[g->parity];
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, c.t, c.dt,
-                       c.step_count, c.mass_out);
+                       c.step_count, c.mass_out, c.mass_src);
     const long long n = (long long)g->cfg.nx * g->cfg.ny;
     int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_init_speed, dim3(blocks), dim3(256), 0, g->stream, d, g->parity);
@@ -82,17 +82,25 @@ int refresh_speed(rkg_grid* g) {
     return RKG_OK;
 }
 
-template <int STAGE>
-void launch_stage(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double tlim) {
+template <int STAGE, bool FRC>
+void launch_stage_f(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double tlim) {
     const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
     if (hllc && nf)
-        hipLaunchKernelGGL((k_stage<STAGE, true, true, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage<STAGE, true, true, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
     else if (hllc)
-        hipLaunchKernelGGL((k_stage<STAGE, true, false, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage<STAGE, true, false, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
     else if (nf)
-        hipLaunchKernelGGL((k_stage<STAGE, false, true, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage<STAGE, false, true, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
     else
-        hipLaunchKernelGGL((k_stage<STAGE, false, false, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage<STAGE, false, false, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+}
+
+template <int STAGE>
+void launch_stage(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double tlim) {
+    if (d.fwx)
+        launch_stage_f<STAGE, true>(g, d, ntiles, mode, tlim);
+    else
+        launch_stage_f<STAGE, false>(g, d, ntiles, mode, tlim);
 }
 
 // Shu-Osher table of a generic scheme, make_timestepper
@@ -120,16 +128,24 @@ SchemeTab scheme_tab(int scheme) {
     return t;
 }
 
-void launch_gen(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, int mode, double tlim) {
+template <bool FRC>
+void launch_gen_f(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, int mode, double tlim) {
     const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
     if (hllc && nf)
-        hipLaunchKernelGGL((k_stage_gen<true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage_gen<true, true, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
     else if (hllc)
-        hipLaunchKernelGGL((k_stage_gen<true, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage_gen<true, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
     else if (nf)
-        hipLaunchKernelGGL((k_stage_gen<false, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage_gen<false, true, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
     else
-        hipLaunchKernelGGL((k_stage_gen<false, false>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+        hipLaunchKernelGGL((k_stage_gen<false, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
+}
+
+void launch_gen(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, int mode, double tlim) {
+    if (d.fwx)
+        launch_gen_f<true>(g, d, gs, ntiles, mode, tlim);
+    else
+        launch_gen_f<false>(g, d, gs, ntiles, mode, tlim);
 }
 
 // Generic explicit SSP steps (euler / rk3 / rk43), solver_advance_2d_generic
@@ -178,13 +194,25 @@ int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
     GridDesc d = make_desc(g);
     const int ntiles = d.nstrips * d.nsegs;
     const double area = d.dx * d.dy;
+    // Interior sum of the precipitation rate for the surface mass source
+    // (fixed over the call: the planes change only between calls).
+    const double* rain = nullptr;
+    if (d.mpart && d.frain) {
+        const int blocks = std::min(MASS_BLOCKS, g->cfg.ny);
+        hipLaunchKernelGGL(k_plane_sum_partial, dim3(blocks), dim3(256), 0, g->stream, d.frain, d.pitch, d.nx, d.ny,
+                           g->mass_buf);
+        hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, g->stream, g->mass_buf, blocks,
+                           g->mass_buf + MASS_BLOCKS + 1);
+        rain = g->mass_buf + MASS_BLOCKS + 1;
+    }
     for (int k = 0; k < n; ++k) {
         if (g->prof) HIP_TRY(hipEventRecord(g->ev[0], g->stream));
         launch_stage<1>(g, d, ntiles, mode, tlim);
         if (g->prof) HIP_TRY(hipEventRecord(g->ev[1], g->stream));
         launch_stage<2>(g, d, ntiles, mode, tlim);
         if (d.mpart)
-            hipLaunchKernelGGL(k_budget, dim3(1), dim3(256), 0, g->stream, g->ctrl, g->parity, d.mpart, ntiles, area);
+            hipLaunchKernelGGL(k_budget, dim3(1), dim3(256), 0, g->stream, g->ctrl, g->parity, d.mpart, ntiles, area,
+                               rain);
         g->parity ^= 1;
         if (g->prof) {
             float a = 0.f, b = 0.f;
@@ -275,7 +303,7 @@ int rkg_create(const rkg_config* cfg, void* stream, rkg_grid** out) {
     hipError_t e = hipMalloc((void**)&g->buf, 7 * g->plane * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&g->ctrl, 2 * sizeof(CtrlSlot));
     if (e == hipSuccess) e = hipHostMalloc((void**)&g->ctrl_host, 2 * sizeof(CtrlSlot), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc((void**)&g->mass_buf, (MASS_BLOCKS + 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&g->mass_buf, (MASS_BLOCKS + 2) * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void**)&g->mass_host, sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(g->buf, 0, 7 * g->plane * sizeof(double), g->stream);
     if (e == hipSuccess) e = hipMemsetAsync(g->ctrl, 0, 2 * sizeof(CtrlSlot), g->stream);
@@ -304,6 +332,7 @@ void rkg_destroy(rkg_grid** pg) {
     if (g->nfield) (void)hipFree(g->nfield);
     if (g->cbuf) (void)hipFree(g->cbuf);
     if (g->mpart) (void)hipFree(g->mpart);
+    if (g->frc) (void)hipFree(g->frc);
     if (g->ctrl) (void)hipFree(g->ctrl);
     if (g->mass_buf) (void)hipFree(g->mass_buf);
     if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
@@ -320,7 +349,7 @@ static int copy_field(rkg_grid* g, int field, double* host_or_dev, int ld, bool 
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     double* f = field_ptr(g, field);
     if (!f) { set_error("unknown or unallocated field %d", field); return RKG_ERR_RANGE; }
-    if (upload && field > RKG_FIELD_B && field != RKG_FIELD_N) { set_error("field %d is read-only", field); return RKG_ERR_RANGE; }
+    if (upload && field > RKG_FIELD_B && field < RKG_FIELD_N) { set_error("field %d is read-only", field); return RKG_ERR_RANGE; }
     if (!host_or_dev || ld < g->cfg.nx) { set_error("bad buffer or leading dimension"); return RKG_ERR_RANGE; }
     const size_t w = (size_t)g->cfg.nx * sizeof(double);
     if (upload) {
@@ -404,7 +433,7 @@ int rkg_set_clock(rkg_grid* g, double t, long long step_count) {
     int rc = read_clock(g, &c);
     if (rc) return rc;
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, t, c.dt, step_count,
-                       c.mass_out);
+                       c.mass_out, c.mass_src);
     HIP_TRY(hipGetLastError());
     g->speed_stale = true;
     return RKG_OK;
@@ -471,7 +500,7 @@ int rkg_set_mass_budget(rkg_grid* g, int enable) {
         int rc = read_clock(g, &c);
         if (rc) return rc;
         hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, g->stream, g->ctrl, g->parity, c.t, c.dt,
-                           c.step_count, 0.0);
+                           c.step_count, 0.0, 0.0);
         HIP_TRY(hipGetLastError());
         g->speed_stale = true;
     } else if (!enable && g->mpart) {
@@ -480,6 +509,40 @@ int rkg_set_mass_budget(rkg_grid* g, int enable) {
         g->mpart = nullptr;
         g->mpart_cap = 0;
     }
+    return RKG_OK;
+}
+
+// Fills a plane with one value.
+__global__ void k_fill(double* p, long long n, double v) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
+}
+
+int rkg_set_forcing(rkg_grid* g, int active) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (active && !g->frc) {
+        HIP_TRY(hipMalloc((void**)&g->frc, 4 * g->plane * sizeof(double)));
+        hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, g->stream, g->frc, (long long)(4 * g->plane), 0.0);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    g->forcing = active != 0;
+    return RKG_OK;
+}
+
+int rkg_get_forcing(rkg_grid* g, int* active) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!active) return RKG_ERR_RANGE;
+    *active = g->forcing ? 1 : 0;
+    return RKG_OK;
+}
+
+int rkg_get_mass_source(rkg_grid* g, double* mass_src) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!mass_src) return RKG_ERR_RANGE;
+    CtrlSlot c;
+    int rc = read_clock(g, &c);
+    if (rc) return rc;
+    *mass_src = c.mass_src;
     return RKG_OK;
 }
 
@@ -505,11 +568,6 @@ int rkg_set_rk_save(rkg_grid* g, int enable) {
         g->save = nullptr;
     }
     return RKG_OK;
-}
-
-// Fills a plane with one value.
-__global__ void k_fill(double* p, long long n, double v) {
-    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
 }
 
 int rkg_use_manning_field(rkg_grid* g, int enable) {

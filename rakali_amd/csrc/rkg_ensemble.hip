@@ -176,7 +176,7 @@ int refresh_speeds(rkg_ensemble* e) {
         const GridDesc d = ens_desc(e, m);
         const CtrlSlot c = e->ctrl_host[2 * m + e->parity];
         hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, e->stream, d.ctrl, e->parity, c.t, c.dt,
-                           c.step_count, c.mass_out);
+                           c.step_count, c.mass_out, c.mass_src);
         const long long cells = (long long)d.nx * d.ny;
         const int blocks = (int)std::min<long long>((cells + 255) / 256, 4096);
         hipLaunchKernelGGL(k_init_speed, dim3(blocks), dim3(256), 0, e->stream, d, e->parity);

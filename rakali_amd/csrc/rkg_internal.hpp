@@ -32,9 +32,11 @@ struct rkg_grid {
     double* mpart;       // per-tile flux_h sums of both stages (mass budget), or null
     long long mpart_cap; // tiles per stage that mpart holds
     int scheme;          // RKG_TIME_*
+    double* frc;         // 4 planes: wind stress x, y, atmospheric pressure, precipitation, or null
+    bool forcing;        // forcing%active: the stage kernels apply the planes in frc
     rk::CtrlSlot* ctrl;      // device [2]
     rk::CtrlSlot* ctrl_host; // pinned [2]
-    double* mass_buf;    // device [MASS_BLOCKS + 1]
+    double* mass_buf;    // device [MASS_BLOCKS + 2]: partial sums, total mass, precipitation sum
     double* mass_host;   // pinned [1]
     int parity;
     int ty;              // rows per stage tile, 0 = chosen from the grid size
@@ -62,6 +64,10 @@ inline double* field_ptr(rkg_grid* g, int field) {
     case RKG_FIELD_HUS: return plane_ptr(g, 4);
     case RKG_FIELD_HVS: return plane_ptr(g, 5);
     case RKG_FIELD_N: return g->nfield;
+    case RKG_FIELD_WIND_X: return g->frc;
+    case RKG_FIELD_WIND_Y: return g->frc ? g->frc + g->plane : nullptr;
+    case RKG_FIELD_PRESSURE: return g->frc ? g->frc + 2 * g->plane : nullptr;
+    case RKG_FIELD_PRECIP: return g->frc ? g->frc + 3 * g->plane : nullptr;
     default: return nullptr;
     }
 }
@@ -109,6 +115,12 @@ inline GridDesc make_desc(rkg_grid* g) {
     d.nsegs = (c.ny + d.ty - 1) / d.ty;
     d.ctrl = g->ctrl;
     d.mpart = g->mpart;
+    if (g->forcing) {
+        d.fwx = g->frc;
+        d.fwy = g->frc + g->plane;
+        d.fpres = g->frc + 2 * g->plane;
+        d.frain = g->frc + 3 * g->plane;
+    }
     d.ssp_coeff = g->scheme == RKG_TIME_RK43 ? 2.0 : 1.0;  // rki_timestepper.F90:60-100
     return d;
 }

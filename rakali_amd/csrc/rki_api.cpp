@@ -578,7 +578,6 @@ bool supported(const Config& c, std::string& msg) {
     if (c.time_scheme == "semi_implicit") { msg = "time_scheme 'semi_implicit' is not implemented here"; return false; }
     if (c.use_multilayer) { msg = "use_multilayer is not implemented here"; return false; }
     if (c.use_nonhydrostatic) { msg = "use_nonhydrostatic is not implemented here"; return false; }
-    if (c.wind_stress_x != 0.0 || c.wind_stress_y != 0.0) { msg = "wind forcing is not implemented here"; return false; }
     if (c.use_linear_bottom_drag) { msg = "use_linear_bottom_drag is not implemented here"; return false; }
     for (int s = 0; s < 4; ++s) {
         if (c.bc[s] == "nested" || c.bc[s] == "chapman") { msg = "boundary '" + c.bc[s] + "' is not implemented here"; return false; }
@@ -624,6 +623,11 @@ struct Sim {
     int nx = 0, ny = 0, ng = 0, nxt = 0, nyt = 0;
     double dx = 0.0, dy = 0.0;
     std::vector<double> h, hu, hv, b, h0, hu0, hv0;
+    // state%forcing (rki_surface_forcing.F90), interior cells only, nx by ny:
+    // the kernels read wind and precipitation at the cell and the pressure
+    // of the four neighbours, whose ghost values no setter writes (0).
+    std::vector<double> wind_x, wind_y, atm_pressure, precipitation, heat_flux, evap_minus_precip;
+    bool forcing_active = false;
     BcParams bc{};
     double manning_n = 0.0, coriolis_f = 0.0;
     bool use_hllc = false, use_ppm_tracer = true;
@@ -961,6 +965,18 @@ int push_state(Sim& s) {
     return 0;
 }
 
+// solver_update_forcing: the forcing planes of the device grid, once it
+// exists and the forcing is on.
+int push_forcing(Sim& s) {
+    if (!s.dev || !s.forcing_active) return 0;
+    int rc = rkg_rc(rkg_set_forcing(s.dev, 1), "rkg_set_forcing");
+    if (!rc) rc = rkg_rc(rkg_upload(s.dev, RKG_FIELD_WIND_X, s.wind_x.data(), s.nx), "rkg_upload");
+    if (!rc) rc = rkg_rc(rkg_upload(s.dev, RKG_FIELD_WIND_Y, s.wind_y.data(), s.nx), "rkg_upload");
+    if (!rc) rc = rkg_rc(rkg_upload(s.dev, RKG_FIELD_PRESSURE, s.atm_pressure.data(), s.nx), "rkg_upload");
+    if (!rc) rc = rkg_rc(rkg_upload(s.dev, RKG_FIELD_PRECIP, s.precipitation.data(), s.nx), "rkg_upload");
+    return rc;
+}
+
 int read_clock(Sim& s) {
     double t, dt;
     long long n;
@@ -1005,6 +1021,11 @@ int create_common(const std::string& text, const std::string& label, void** hand
     s->dy = c.dy;
     const size_t n = (size_t)s->nxt * (size_t)s->nyt;
     for (std::vector<double>* f : {&s->h, &s->hu, &s->hv, &s->b, &s->h0, &s->hu0, &s->hv0}) f->assign(n, 0.0);
+    const size_t ni = (size_t)c.nx * (size_t)c.ny;
+    s->wind_x.assign(ni, c.wind_stress_x);   // uniform fallback, rki_state.F90:272-274
+    s->wind_y.assign(ni, c.wind_stress_y);
+    for (std::vector<double>* f : {&s->atm_pressure, &s->precipitation, &s->heat_flux, &s->evap_minus_precip})
+        f->assign(ni, 0.0);
     s->manning_n = c.manning_n;
     s->coriolis_f = c.coriolis_f;
     s->use_hllc = c.use_hllc;
@@ -1121,6 +1142,7 @@ int rki_finalize(void* handle) {
     if (!rc) rc = rkg_rc(rkg_set_time_scheme(s->dev, time_scheme_tag(s->cfg.time_scheme)), "rkg_set_time_scheme");
     if (!rc) rc = rkg_rc(rkg_set_clock(s->dev, s->t_current, 0), "rkg_set_clock");
     if (!rc) rc = upload_all(*s);
+    if (!rc) rc = push_forcing(*s);
     if (rc) {
         if (s->dev) rkg_destroy(&s->dev);
         return rc;
@@ -1286,29 +1308,39 @@ int rki_set_h(void* h, const double* d, int nx, int ny) { return set_state_field
 int rki_set_hu(void* h, const double* d, int nx, int ny) { return set_state_field(h, &Sim::hu, d, nx, ny, false); }
 int rki_set_hv(void* h, const double* d, int nx, int ny) { return set_state_field(h, &Sim::hv, d, nx, ny, false); }
 
-static int forcing_unsupported(void* handle, const double* a, int nx, int ny, const char* what) {
+// Stores one forcing field (rki_api.F90:896-1034).  `activate` is the
+// setters of wind, pressure and precipitation, which set
+// state%forcing%active; heat flux and E-P only feed the multilayer path
+// and leave the 2-D step unchanged.
+static int set_forcing_field(void* handle, std::vector<double> Sim::*field, const double* src, int nx, int ny,
+                             bool activate) {
     Sim* s = resolve(handle);
-    int rc = field_check(s, a, nx, ny);
+    int rc = field_check(s, src, nx, ny);
     if (rc) return rc;
-    set_error("%s forcing is not implemented here", what);
-    return RKG_ERR_UNSUPPORTED;
+    std::memcpy((s->*field).data(), src, (size_t)nx * (size_t)ny * sizeof(double));
+    if (activate) s->forcing_active = true;
+    return push_forcing(*s);
 }
 
 int rki_set_wind(void* h, const double* wx, const double* wy, int nx, int ny) {
-    if (!wy) return resolve(h) ? -2 : -100;
-    return forcing_unsupported(h, wx, nx, ny, "wind");
+    Sim* s = resolve(h);
+    int rc = field_check(s, wx, nx, ny);
+    if (!rc) rc = field_check(s, wy, nx, ny);
+    if (rc) return rc;
+    std::memcpy(s->wind_x.data(), wx, (size_t)nx * (size_t)ny * sizeof(double));
+    return set_forcing_field(h, &Sim::wind_y, wy, nx, ny, true);
 }
 int rki_set_atm_pressure(void* h, const double* p, int nx, int ny) {
-    return forcing_unsupported(h, p, nx, ny, "atmospheric pressure");
+    return set_forcing_field(h, &Sim::atm_pressure, p, nx, ny, true);
 }
 int rki_set_precipitation(void* h, const double* p, int nx, int ny) {
-    return forcing_unsupported(h, p, nx, ny, "precipitation");
+    return set_forcing_field(h, &Sim::precipitation, p, nx, ny, true);
 }
 int rki_set_heat_flux(void* h, const double* q, int nx, int ny) {
-    return forcing_unsupported(h, q, nx, ny, "heat flux");
+    return set_forcing_field(h, &Sim::heat_flux, q, nx, ny, false);
 }
 int rki_set_evap_minus_precip(void* h, const double* e, int nx, int ny) {
-    return forcing_unsupported(h, e, nx, ny, "evaporation minus precipitation");
+    return set_forcing_field(h, &Sim::evap_minus_precip, e, nx, ny, false);
 }
 
 static int tracer_set(void* handle) {

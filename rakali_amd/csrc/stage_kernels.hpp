@@ -61,7 +61,7 @@ __device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, i
 // (strip, segment, leader); the leader is the single tile per grid and
 // launch that advances the control slot.  STAGE 1 and 2 are the fused
 // SSP-RK2 stages; STAGE 3 is one stage `gs` of a generic SSP scheme.
-template <int STAGE, bool HLLC, bool NF, bool REC, class Where>
+template <int STAGE, bool HLLC, bool NF, bool REC, class Where, bool FRC = false>
 __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p, int mode, double tlim,
                                            Where where, const GenStage& gs = GenStage{}) {
     const int3 w = where(d);
@@ -81,6 +81,7 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
             q->step_count = active ? cp.step_count + 1 : cp.step_count;
             q->speed_bits = active ? 0ull : cp.speed_bits;
             q->mass_out = cp.mass_out;
+            q->mass_src = cp.mass_src;
             q->active = active ? 1 : 0;
             if (REC && d.hist != nullptr) {   // the step's clock for a reverse sweep (rki_solver.F90:731)
                 d.hist[0] = cp.t;
@@ -128,9 +129,9 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     const int strip = w.x, seg = w.y;
     double smax, fsum;
     if (tile_is_interior(d, strip, seg))
-        smax = process_tile<STAGE, HLLC, false, NF, REC>(d, s, L, strip, seg, fsum);
+        smax = process_tile<STAGE, HLLC, false, NF, REC, FRC>(d, s, L, strip, seg, fsum);
     else
-        smax = process_tile<STAGE, HLLC, true, NF, REC>(d, s, L, strip, seg, fsum);
+        smax = process_tile<STAGE, HLLC, true, NF, REC, FRC>(d, s, L, strip, seg, fsum);
     if (STAGE == 2 || STAGE == 3) {
         smax = wave_max(smax);
         if (threadIdx.x == 0 && smax > 0.0)
@@ -144,34 +145,38 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
 }
 
 // Single grid: one workgroup (one wavefront) per tile.
-template <int STAGE, bool HLLC, bool NF, bool REC>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage(GridDesc d, int p, int mode, double tlim) {
-    __shared__ TileLds L;
-    stage_tile<STAGE, HLLC, NF, REC>(d, L, p, mode, tlim, [](const GridDesc& g) {
+struct SingleGridTile {
+    __device__ int3 operator()(const GridDesc& g) const {
         const int tile = xcd_tile(blockIdx.x, g.nstrips * g.nsegs);
         return make_int3(tile % g.nstrips, tile / g.nstrips, blockIdx.x == 0);
-    });
+    }
+};
+
+// FRC: with the surface forcing (forcing_active).
+template <int STAGE, bool HLLC, bool NF, bool REC, bool FRC = false>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage(GridDesc d, int p, int mode, double tlim) {
+    __shared__ TileLds L;
+    stage_tile<STAGE, HLLC, NF, REC, SingleGridTile, FRC>(d, L, p, mode, tlim, SingleGridTile{});
 }
 
 // One stage of a generic SSP scheme, single grid.
-template <bool HLLC, bool NF>
+template <bool HLLC, bool NF, bool FRC = false>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage_gen(GridDesc d, GenStage gs, int p, int mode, double tlim) {
     __shared__ TileLds L;
-    stage_tile<3, HLLC, NF, false>(d, L, p, mode, tlim, [](const GridDesc& g) {
-        const int tile = xcd_tile(blockIdx.x, g.nstrips * g.nsegs);
-        return make_int3(tile % g.nstrips, tile / g.nstrips, blockIdx.x == 0);
-    }, gs);
+    stage_tile<3, HLLC, NF, false, SingleGridTile, FRC>(d, L, p, mode, tlim, SingleGridTile{}, gs);
 }
 
 // Resets control slot p to time t / step count n with an empty speed max;
 // the mass budget is set to mass_out.
-__global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long long n, double mass_out) {
+__global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long long n, double mass_out,
+                             double mass_src) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctrl[p].t = t;
         ctrl[p].dt = dt;
         ctrl[p].step_count = n;
         ctrl[p].speed_bits = 0ull;
         ctrl[p].mass_out = mass_out;
+        ctrl[p].mass_src = mass_src;
         ctrl[p].active = 1;
     }
 }
@@ -179,9 +184,13 @@ __global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long lo
 // Boundary mass budget of one SSP-RK2 step (rki_solver.F90:507-537, called
 // after each stage's flux at :769 and :796): mass_out -= 0.5 dt sum(flux_h)
 // area per stage.  The per-tile sums of both stages are added in a fixed
-// order (strided per thread, then a tree).  Slot 1-p is the step's result.
+// order (strided per thread, then a tree).  With the surface forcing on,
+// `rain` holds the interior sum of the precipitation rate, and mass_src +=
+// 0.5 dt rain area per stage (accumulate_mass_src, rki_solver.F90:539-606,
+// called after each fused stage at :783 and :810).  Slot 1-p is the step's
+// result.
 __global__ __launch_bounds__(256) void k_budget(CtrlSlot* ctrl, int p, const double* mpart, int ntiles,
-                                                double area) {
+                                                double area, const double* rain) {
     __shared__ double red[2][256];
     if (!ctrl[1 - p].active) return;
     double a1 = 0.0, a2 = 0.0;
@@ -206,6 +215,13 @@ __global__ __launch_bounds__(256) void k_budget(CtrlSlot* ctrl, int p, const dou
         m = m - 0.5 * dt * red[0][0] * area;
         m = m - 0.5 * dt * red[1][0] * area;
         q->mass_out = m;
+        if (rain != nullptr) {
+            const double r = *rain;
+            double src = q->mass_src;
+            src = src + 0.5 * dt * r * area;
+            src = src + 0.5 * dt * r * area;
+            q->mass_src = src;
+        }
     }
 }
 
@@ -258,6 +274,23 @@ __global__ __launch_bounds__(256) void k_mass_partial(GridDesc d, double* partia
             if (h > 0.0 && h < 1.0e6) acc = acc + h * area;
         }
     }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Per-block sums of the interior of one plane (rows strided over blocks),
+// added up by k_mass_final: a fixed order, so the result is deterministic.
+__global__ __launch_bounds__(256) void k_plane_sum_partial(const double* f, long long pitch, int nx, int ny,
+                                                           double* partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int j = blockIdx.x; j < ny; j += gridDim.x)
+        for (int i = threadIdx.x; i < nx; i += 256) acc = acc + f[(long long)j * pitch + i];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {

@@ -29,6 +29,7 @@ constexpr double DRY_TOLERANCE = 1.0e-6;
 constexpr double THIN_LAYER_THRESHOLD = 1.0e-3;
 constexpr double DESING_EPS = 1.5e-4;
 constexpr double FROUDE_BACKSTOP = 50.0;
+constexpr double RHO_WATER = 1035.0;          // rki_constants.F90:40
 constexpr double TWO_PI = 6.283185307179586;  // 8*atan(1), rki_constants.F90:114
 constexpr double SQRT2 = 1.4142135623730951;  // sqrt(2.0_wp) folded
 
@@ -302,12 +303,20 @@ struct SourceTerms {
     double coriolis_f;  // f (used by the reverse sweep's d/d dt)
 };
 
-// Euler update + clamp + Manning + Coriolis, rki_kernel_fused.F90:228-290
-// (forcing inactive).  With NF the Manning coefficient is the cell's own
-// n_cell (friction where n_cell > 0) instead of the scalar one.
-template <bool EXACT, bool NF>
+// Surface forcing at one cell (rki_surface_forcing.F90): wind stress (Pa),
+// the central pressure gradient (Pa/m) and the precipitation rate (m/s).
+struct Frc {
+    double wx, wy, dpdx, dpdy, rain;
+};
+
+// Euler update + clamp + Manning + forcing + Coriolis + precipitation,
+// rki_kernel_fused.F90:228-290.  With NF the Manning coefficient is the
+// cell's own n_cell (friction where n_cell > 0) instead of the scalar one;
+// FRC applies the surface forcing f (forcing_active).
+template <bool EXACT, bool NF, bool FRC = false>
 __device__ __forceinline__ void stage_body(const SourceTerms& s, double dt, double fh, double fhu,
-                                           double fhv, double& h, double& hu, double& hv, double n_cell) {
+                                           double fhv, double& h, double& hu, double& hv, double n_cell,
+                                           const Frc& f = Frc{}) {
     h = h + dt * fh;
     hu = hu + dt * fhu;
     hv = hv + dt * fhv;
@@ -325,12 +334,19 @@ __device__ __forceinline__ void stage_body(const SourceTerms& s, double dt, doub
             hu = rdiv<EXACT>(hu, rd);
             hv = rdiv<EXACT>(hv, rd);
         }
+        if (FRC) {   // rki_kernel_fused.F90:273-280
+            hu = hu + dt * f.wx / RHO_WATER;
+            hv = hv + dt * f.wy / RHO_WATER;
+            hu = hu - dt * h * f.dpdx / RHO_WATER;
+            hv = hv - dt * h * f.dpdy / RHO_WATER;
+        }
         if (s.has_coriolis) {
             const double hu_old = hu, hv_old = hv;
             hu = hu_old * s.cosf + hv_old * s.sinf;
             hv = -hu_old * s.sinf + hv_old * s.cosf;
         }
     }
+    if (FRC) h = h + dt * f.rain;   // every cell, :286-289
 }
 
 // Per-cell CFL speed term, rki_kernel_timestep.F90:56-69; 0 for dry cells.

@@ -27,7 +27,8 @@ struct alignas(64) CtrlSlot {
     long long step_count;         // state%step_count
     double mass_out;              // boundary mass budget (rki_solver.F90:507-537)
     int active;                   // 1 if the step that wrote this slot ran
-    int pad[3];
+    int pad;
+    double mass_src;              // surface mass source (accumulate_mass_src, rki_solver.F90:539-606)
 };
 
 struct GridDesc {
@@ -43,6 +44,8 @@ struct GridDesc {
     double* rec;                  // 3 planes receiving each cell's flux divergence, or null
     double* mpart;                // per-tile sums of flux_h, [stage 1 tiles | stage 2 tiles], or null
     double* hist;                 // recording stage 1: (t, dt, dt recomputed) of the step, or null
+    const double* fwx; const double* fwy;   // surface forcing planes (FRC kernels): wind stress,
+    const double* fpres; const double* frain; // atmospheric pressure, precipitation
     long long plane;              // elements per plane (pitch * ny)
     double ssp_coeff;             // stable-dt multiplier of the time scheme (rki_timestepper.F90)
     double g, cfl, manning_n, coriolis_f;
@@ -273,7 +276,7 @@ struct ExactTag {
 // shared-reciprocal ones; both give the IEEE quotients.
 // STAGE 1 writes U* to B; STAGE 2 reads U* and U^n, applies the RK2 average
 // and writes U^{n+1} over A, returning the lane's max CFL speed.
-template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC>
+template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC, bool FRC = false>
 __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx& s, TileLds& L,
                                                int strip, int seg, double& fsum) {
     const int lane = threadIdx.x;
@@ -390,10 +393,22 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                     d.rec[k] = fh; d.rec[d.plane + k] = fhu; d.rec[2 * d.plane + k] = fhv;
                 }
                 double h = rc.h, hu = rc.hu, hv = rc.hv;
+                Frc fr{0.0, 0.0, 0.0, 0.0, 0.0};
+                if (FRC) {   // pressure off the grid is the ghost value, which no setter writes: 0
+                    fr.wx = d.fwx[k];
+                    fr.wy = d.fwy[k];
+                    const double pe = c + 1 < d.nx ? d.fpres[k + 1] : 0.0;
+                    const double pw = c > 0 ? d.fpres[k - 1] : 0.0;
+                    const double pn = j + 1 < d.ny ? d.fpres[k + pitch] : 0.0;
+                    const double ps = j > 0 ? d.fpres[k - pitch] : 0.0;
+                    fr.dpdx = (pe - pw) / (2.0 * d.dx);
+                    fr.dpdy = (pn - ps) / (2.0 * d.dy);
+                    fr.rain = d.frain[k];
+                }
                 if (NF)
-                    stage_body<EX, true>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, d.nfield[k]);
+                    stage_body<EX, true, FRC>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, d.nfield[k], fr);
                 else
-                    stage_body<EX, false>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, 0.0);
+                    stage_body<EX, false, FRC>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, 0.0, fr);
                 if (STAGE == 2) {
                     // rki_kernel_fused.F90:435-463
                     h = 0.5 * (un.h + h);

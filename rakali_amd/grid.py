@@ -144,6 +144,27 @@ class Grid:
         check("rkg_get_mass_budget", self._lib.rkg_get_mass_budget(self.handle, ctypes.byref(m)))
         return m.value
 
+    @property
+    def forcing(self) -> bool:
+        """Surface forcing on: state%forcing%active.  Switching it on the first
+        time allocates the zeroed fields wind_x, wind_y (wind stress, Pa),
+        atm_pressure (Pa) and precipitation (m/s)."""
+        a = ctypes.c_int()
+        check("rkg_get_forcing", self._lib.rkg_get_forcing(self.handle, ctypes.byref(a)))
+        return bool(a.value)
+
+    @forcing.setter
+    def forcing(self, active: bool) -> None:
+        check("rkg_set_forcing", self._lib.rkg_set_forcing(self.handle, int(bool(active))))
+
+    @property
+    def mass_src(self) -> float:
+        """Volume added by precipitation since the mass budget was enabled
+        (accumulate_mass_src, rki_solver.F90:539-606)."""
+        m = ctypes.c_double()
+        check("rkg_get_mass_source", self._lib.rkg_get_mass_source(self.handle, ctypes.byref(m)))
+        return m.value
+
     def device_ptr(self, name: str):
         p = ctypes.c_void_p()
         pitch = ctypes.c_longlong()

@@ -22,8 +22,10 @@ def ghosted(a: np.ndarray, ng: int) -> np.ndarray:
     return out
 
 
-def run_oracle(scn, steps=None, ng=2, mode="step", target=None, **override):
-    """Run the CPU restatement on a scenario; returns (sim, fields dict)."""
+def run_oracle(scn, steps=None, ng=2, mode="step", target=None, forcing=None, **override):
+    """Run the CPU restatement on a scenario; returns (sim, fields dict).
+    `forcing` maps wind_x / wind_y / atm_pressure / precipitation to
+    interior (nx, ny) arrays; the ghost cells stay 0 and the forcing is on."""
     import oracle as O
 
     kw = dict(scn.grid_kw)
@@ -37,6 +39,15 @@ def run_oracle(scn, steps=None, ng=2, mode="step", target=None, **override):
         for n, (a, ph, om) in enumerate(tcs):
             p.tc_amp[n], p.tc_phase[n], p.tc_omega[n] = a, ph, om
     sim = O.Sim(p)
+    if forcing is not None:
+        sim.forcing_arrays = {}
+        for name in ("wind_x", "wind_y", "atm_pressure", "precipitation"):
+            a = O.zeros(p)
+            if name in forcing:
+                a[ng:ng + scn.nx, ng:ng + scn.ny] = forcing[name]
+            sim.forcing_arrays[name] = a
+            setattr(p, name, a.ctypes.data)
+        p.forcing_active = 1
     sim.set_interior("b", scn.b)
     sim.set_interior("h", scn.h)
     sim.set_interior("hu", scn.hu)

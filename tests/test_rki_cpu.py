@@ -97,7 +97,6 @@ def test_namelist_syntax_forms(L):
     ("&time_nml\n time_scheme = 'semi_implicit'\n/", -5, "semi_implicit"),
     ("&boundary_nml\n bc_north = 'chapman'\n/", -5, "chapman"),
     ("&boundary_nml\n bc_north = 'sponge', sponge_width = 4\n/", -5, "sponge"),
-    ("&physics_nml\n wind_stress_x = 0.1\n/", -5, "wind"),
     ("&grid_nml\n nghost = 1\n/", -5, "nghost"),
     ("&grid_nml\n nx = 8, ny = 1\n/", -5, "ny must be"),
 ])
@@ -156,11 +155,15 @@ def test_phase1_fields_and_setters(L):
     assert (nxt.value, nyt.value) == (10, 8)
     full = np.ctypeslib.as_array((ctypes.c_double * 80).from_address(ptr.value)).reshape(10, 8, order="F")
     assert np.array_equal(full[2:8, 2:6], b)
-    # forcing fields and tracers: not implemented (-5) / multilayer inactive (-3)
+    # forcing fields (stored until the device grid exists), tracers:
+    # multilayer inactive (-3)
     z = np.zeros((6, 4), order="F")
-    assert L.rki_set_wind(h, z.ctypes.data_as(pd), z.ctypes.data_as(pd), 6, 4) == -5
-    assert L.rki_set_precipitation(h, z.ctypes.data_as(pd), 6, 4) == -5
+    assert L.rki_set_wind(h, z.ctypes.data_as(pd), z.ctypes.data_as(pd), 6, 4) == 0
+    assert L.rki_set_wind(h, z.ctypes.data_as(pd), None, 6, 4) == -2
+    assert L.rki_set_precipitation(h, z.ctypes.data_as(pd), 6, 4) == 0
     assert L.rki_set_precipitation(h, z.ctypes.data_as(pd), 5, 4) == -2
+    assert L.rki_set_heat_flux(h, z.ctypes.data_as(pd), 6, 4) == 0
+    assert L.rki_set_evap_minus_precip(h, z.ctypes.data_as(pd), 6, 4) == 0
     assert L.rki_set_salinity(h, z.ctypes.data_as(pd), 6, 4, 1) == -3
     nz = ctypes.c_int()
     assert L.rki_get_h_layer_ptr(h, ctypes.byref(ptr), ctypes.byref(nxt), ctypes.byref(nyt), ctypes.byref(nz)) == -1
@@ -286,8 +289,9 @@ def test_two_phase_class_without_stepping():
         sim.set_initial_h(np.ones((8, 16)))
     with pytest.raises(RuntimeError, match="not finalised"):
         sim.step()
-    with pytest.raises(RuntimeError):
-        sim.set_wind(np.zeros((16, 8)), np.zeros((16, 8)))
+    sim.set_wind(np.zeros((16, 8)), np.zeros((16, 8)))
+    with pytest.raises(ValueError):
+        sim.set_precipitation(np.zeros((8, 16)))
     with pytest.raises(RuntimeError, match="multilayer"):
         sim.set_salinity(np.zeros((16, 8, 1)))
     with pytest.raises(ValueError):
