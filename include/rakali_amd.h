@@ -197,8 +197,21 @@ int rkg_get_forcing(rkg_grid *g, int *active);
 /* Surface mass source state%barotropic%mass_src (accumulate_mass_src,
  * rki_solver.F90:539-606): while the mass budget is enabled and the
  * forcing is on, each SSP-RK2 step adds 0.5*dt*sum(precipitation)*dx*dy
- * per stage.  rkg_set_mass_budget(g, 1) starts it at 0. */
+ * per stage, and the sponge's change of volume (rkg_set_sponge).
+ * rkg_set_mass_budget(g, 1) starts it at 0. */
 int rkg_get_mass_source(rkg_grid *g, double *mass_src);
+/* Sponge relaxation (apply_sponge, rki_kernel_sponge.F90:16-88; namelist
+ * sponge_width, sponge_strength, sponge_h_ref, bc_params_t
+ * rki_boundary_types.F90:73-77).  Interior cells closer than `width`
+ * cells to an RKG_BC_SPONGE edge relax toward (h_ref, 0, 0) after every
+ * stage: h -= dt*alpha*(h - h_ref), hu -= dt*alpha*hu, hv likewise, with
+ * alpha = strength*((width - d)/width)^2 and d the distance in cells from
+ * the nearest sponge edge.  Off for width 0 (the default) or strength <= 0.
+ * With the mass budget on, an SSP-RK2 step adds 0.5*(sum of the change of
+ * h)*dx*dy per stage to the mass source (accumulate_sponge_mass_src,
+ * rki_solver.F90:608-657).  The adjoint does not support the sponge. */
+int rkg_set_sponge(rkg_grid *g, int width, double strength, double h_ref);
+int rkg_get_sponge(rkg_grid *g, int *width, double *strength, double *h_ref);
 /* Reads / replaces the scalar configuration (nx and ny cannot change):
  * boundary values, tide, friction, CFL, t_end, HLL/HLLC. */
 int rkg_get_config(rkg_grid *g, rkg_config *cfg);

@@ -54,6 +54,8 @@ class Params(ctypes.Structure):
         ("wind_x", ctypes.c_void_p), ("wind_y", ctypes.c_void_p),
         ("atm_pressure", ctypes.c_void_p), ("precipitation", ctypes.c_void_p),
         ("forcing_active", ctypes.c_int),
+        ("sponge_width", ctypes.c_int),
+        ("sponge_strength", ctypes.c_double), ("sponge_h_ref", ctypes.c_double),
     ]
 
 
@@ -87,6 +89,8 @@ def lib():
         L.orc_compute_flux.argtypes = [P] + [dp] * 7 + [ctypes.c_int]
         L.orc_fused_stage1.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
         L.orc_fused_stage2.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
+        L.orc_apply_sponge.argtypes = [P, dp, dp, dp, ctypes.c_double]
+        L.orc_fused_stage_generic.argtypes = [P] + [dp] * 6 + [ctypes.c_double]
         L.orc_compute_timestep.argtypes = [P, dp, dp, dp]
         L.orc_compute_timestep.restype = ctypes.c_double
         L.orc_tidal_eta.argtypes = [P, ctypes.c_double]
@@ -143,6 +147,15 @@ def fused_stage1(p, h, hu, hv, fh, fhu, fhv, dt):
                            _ptr(h0), _ptr(hu0), _ptr(hv0),
                            _ptr(fh), _ptr(fhu), _ptr(fhv), dt)
     return h0, hu0, hv0
+
+
+def apply_sponge(p, h, hu, hv, dt):
+    lib().orc_apply_sponge(ctypes.byref(p), _ptr(h), _ptr(hu), _ptr(hv), dt)
+
+
+def fused_stage_generic(p, h, hu, hv, fh, fhu, fhv, dt):
+    lib().orc_fused_stage_generic(ctypes.byref(p), _ptr(h), _ptr(hu), _ptr(hv),
+                                  _ptr(fh), _ptr(fhu), _ptr(fhv), dt)
 
 
 def fused_stage2(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt):

@@ -577,6 +577,39 @@ void orc_ssp_combine(const orc_params *p, double *h, double *hu, double *hv,
     }
 }
 
+/* apply_sponge, rki_kernel_sponge.F90:16-88: interior cells closer than
+ * sponge_width cells to a BC_SPONGE edge relax toward (h_ref, 0, 0) with
+ * alpha = strength ((width - d) / width)^2, d the distance in cells from
+ * the nearest such edge (0 for the first interior cell). */
+void orc_apply_sponge(const orc_params *p, double *h, double *hu, double *hv, double dt)
+{
+    const int nxt = NT(p), nyt = MT(p), ng = p->ng;
+    const int sw = p->sponge_width;
+    if (sw <= 0) return;
+    const double strength = p->sponge_strength, h_ref = p->sponge_h_ref;
+    if (strength <= 0.0) return;
+    const int i_first = ng + 1, i_last = nxt - ng, j_first = ng + 1, j_last = nyt - ng;
+    const int has_w = p->bc[0] == ORC_BC_SPONGE, has_e = p->bc[1] == ORC_BC_SPONGE;
+    const int has_s = p->bc[2] == ORC_BC_SPONGE, has_n = p->bc[3] == ORC_BC_SPONGE;
+    for (int j = j_first; j <= j_last; ++j) {
+        for (int i = i_first; i <= i_last; ++i) {
+            int d_min = sw + 1;
+            if (has_w && i - i_first < sw && i - i_first < d_min) d_min = i - i_first;
+            if (has_e && i_last - i < sw && i_last - i < d_min) d_min = i_last - i;
+            if (has_s && j - j_first < sw && j - j_first < d_min) d_min = j - j_first;
+            if (has_n && j_last - j < sw && j_last - j < d_min) d_min = j_last - j;
+            if (d_min < sw) {
+                const double ratio = (double)(sw - d_min) / (double)sw;
+                const double alpha = strength * ratio * ratio;
+                const size_t k = IX(i, j);
+                h[k] = h[k] - dt * alpha * (h[k] - h_ref);
+                hu[k] = hu[k] - dt * alpha * hu[k];
+                hv[k] = hv[k] - dt * alpha * hv[k];
+            }
+        }
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* CFL timestep (rki_kernel_timestep.F90:16-78)                           */
 /* ---------------------------------------------------------------------- */
@@ -798,6 +831,18 @@ static double interior_sum(const orc_params *p, const double *f)
     return acc;
 }
 
+/* accumulate_sponge_mass_src, rki_solver.F90:608-657: the sponge, and the
+ * change of the interior sum of h it made, weighted 0.5 per RK2 stage. */
+static void sponge_with_budget(const orc_params *p, orc_clock *clk, double *h, double *hu,
+                               double *hv, double dt, double area)
+{
+    if (p->sponge_width <= 0) return;   /* apply_sponge is a no-op then */
+    const double before = interior_sum(p, h);
+    orc_apply_sponge(p, h, hu, hv, dt);
+    const double after = interior_sum(p, h);
+    clk->mass_src = clk->mass_src + 0.5 * (after - before) * area;
+}
+
 /* ---------------------------------------------------------------------- */
 /* One SSP-RK2 step: rki_solver.F90:659-815 (2-D barotropic fused path)   */
 /* driven by rki_step / rki_advance (rki_api.F90:358-410).                */
@@ -867,6 +912,7 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
                 orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);
                 orc_fused_stage_generic(p, h, hu, hv, fh, fhu, fhv, dt_stage);
                 if (alpha[st] != 0.0) orc_ssp_combine(p, h, hu, hv, h0, hu0, hv0, alpha[st]);
+                orc_apply_sponge(p, h, hu, hv, dt_stage);   /* :919-920 */
             }
             clk->t = clk->t + dt;
             ++taken;
@@ -879,6 +925,7 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
         orc_fused_stage1(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt);
         if (p->forcing_active && p->precipitation)   /* accumulate_mass_src :569-601 */
             clk->mass_src = clk->mass_src + 0.5 * dt * interior_sum(p, p->precipitation) * area;
+        sponge_with_budget(p, clk, h, hu, hv, dt, area);   /* :787 */
         /* stage 2 :789-814, bc%time = time + dt */
         double time2 = time + dt;
         orc_apply_bc(p, time2, h, hu, hv, b);
@@ -887,6 +934,7 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
         orc_fused_stage2(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt);
         if (p->forcing_active && p->precipitation)
             clk->mass_src = clk->mass_src + 0.5 * dt * interior_sum(p, p->precipitation) * area;
+        sponge_with_budget(p, clk, h, hu, hv, dt, area);   /* :814 */
         /* rki_api.F90:378 */
         clk->t = clk->t + dt;
         ++taken;

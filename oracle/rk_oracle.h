@@ -45,6 +45,10 @@ typedef struct {
      * (rki_kernel_fused.F90:268-283, :286-289). */
     const double *wind_x, *wind_y, *atm_pressure, *precipitation;
     int forcing_active;
+    /* Sponge relaxation next to BC_SPONGE edges (bc_params_t,
+     * rki_boundary_types.F90:73-77); off when width <= 0 or strength <= 0. */
+    int sponge_width;
+    double sponge_strength, sponge_h_ref;
 } orc_params;
 
 /* Explicit SSP schemes, tags as rki_timestepper.F90:27-33.  ORC_TIME_RK2
@@ -84,6 +88,8 @@ void orc_fused_stage_generic(const orc_params *p, double *h, double *hu, double 
 void orc_ssp_combine(const orc_params *p, double *h, double *hu, double *hv,
                      const double *h0, const double *hu0, const double *hv0,
                      double a_n);
+/* apply_sponge (rki_kernel_sponge.F90:16-88). */
+void orc_apply_sponge(const orc_params *p, double *h, double *hu, double *hv, double dt);
 double orc_compute_timestep(const orc_params *p, const double *h,
                             const double *hu, const double *hv);
 double orc_tidal_eta(const orc_params *p, double time);

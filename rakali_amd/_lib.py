@@ -97,6 +97,9 @@ SIGNATURES = {
     "rkg_set_forcing": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_get_forcing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     "rkg_get_mass_source": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "rkg_set_sponge": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "rkg_get_sponge": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]),
     "rkg_get_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_set_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),
     "rkg_ensemble_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, _P, ctypes.POINTER(_P)]),

@@ -14,15 +14,19 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "librakali_amd.so"
-SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip", CSRC / "rkg_adjoint.hip"]
+SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip", CSRC / "rkg_adjoint.hip",
+           *(CSRC / f"stage_feat{k}.hip" for k in range(4))]
 HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp", CSRC / "adjoint.hpp",
-           CSRC / "stage_kernels.hpp", CSRC / "rkg_internal.hpp", ROOT / "include" / "rakali_amd.h"]
+           CSRC / "stage_kernels.hpp", CSRC / "stage_launch.hpp", CSRC / "stage_feat.inc",
+           CSRC / "rkg_internal.hpp", ROOT / "include" / "rakali_amd.h"]
 RKI_LIB = PKG / "librakali.so"
 RKI_SOURCES = [CSRC / "rki_api.cpp"]
 RKI_HEADERS = [CSRC / "crmath.hpp", ROOT / "include" / "rakali.h", ROOT / "include" / "rakali_amd.h"]
@@ -87,9 +91,31 @@ def _run(cmd: list[str], out: Path, verbose: bool) -> Path:
     return out
 
 
+def _compile_objects(sources: list[Path], outdir: Path, verbose: bool) -> list[str]:
+    """Compiles each translation unit to an object, the units in parallel
+    (each one instantiates many stage kernels and takes tens of seconds)."""
+    flags = [f for f in compile_flags() if f != "-shared"]
+
+    def one(src: Path) -> str:
+        obj = outdir / (src.stem + ".o")
+        cmd = [hipcc(), *flags, "-c", str(src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src.name} ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+        return str(obj)
+
+    jobs = max(1, min(len(sources), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as pool:
+        return list(pool.map(one, sources))
+
+
 def build(verbose: bool = False) -> Path:
     """Compile both libraries unconditionally; return the step library's path."""
-    _run([hipcc(), *compile_flags(), *map(str, SOURCES)], LIB, verbose)
+    with tempfile.TemporaryDirectory(prefix="rakali_amd_build_") as tmp:
+        objs = _compile_objects(SOURCES, Path(tmp), verbose)
+        _run([hipcc(), *compile_flags(), *objs], LIB, verbose)
     # The handle API is host code (it uses the step's host-side sin/cos);
     # it links the step library by soname and finds it next to itself.
     _run([hipcc(), *compile_flags(), "-x", "hip", *map(str, RKI_SOURCES),

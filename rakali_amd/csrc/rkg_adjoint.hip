@@ -736,6 +736,10 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
         set_error("the adjoint does not cover the surface forcing; switch it off");
         return RKG_ERR_UNSUPPORTED;
     }
+    if (sponge_on(g)) {
+        set_error("the adjoint does not cover the sponge relaxation; set its width to 0");
+        return RKG_ERR_UNSUPPORTED;
+    }
     int rc = rkg_use_manning_field(g, 1);
     if (rc) return rc;
     rkg_adjoint* a = new rkg_adjoint();
@@ -858,6 +862,10 @@ int fwd_begin(rkg_adjoint* a) {
     }
     if (g->forcing) {
         set_error("the adjoint does not cover the surface forcing; switch it off");
+        return RKG_ERR_UNSUPPORTED;
+    }
+    if (sponge_on(g)) {
+        set_error("the adjoint does not cover the sponge relaxation; set its width to 0");
         return RKG_ERR_UNSUPPORTED;
     }
     HIP_TRY(hipMemsetAsync(a->loss_dev, 0, sizeof(double), g->stream));

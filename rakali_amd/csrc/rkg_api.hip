@@ -22,6 +22,7 @@
 #include "../../include/rakali_amd.h"
 #include "rkg_internal.hpp"
 #include "stage_kernels.hpp"
+#include "stage_launch.hpp"
 
 using namespace rk;
 
@@ -82,25 +83,26 @@ int refresh_speed(rkg_grid* g) {
     return RKG_OK;
 }
 
-template <int STAGE, bool FRC>
-void launch_stage_f(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double tlim) {
-    const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
-    if (hllc && nf)
-        hipLaunchKernelGGL((k_stage<STAGE, true, true, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-    else if (hllc)
-        hipLaunchKernelGGL((k_stage<STAGE, true, false, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-    else if (nf)
-        hipLaunchKernelGGL((k_stage<STAGE, false, true, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
-    else
-        hipLaunchKernelGGL((k_stage<STAGE, false, false, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, g->parity, mode, tlim);
+// Source terms a descriptor asks the stage kernels for.
+int feat_of(const GridDesc& d) {
+    return (d.fwx ? FEAT_FORCING : 0) | (d.sponge_width > 0 ? FEAT_SPONGE : 0);
 }
 
 template <int STAGE>
 void launch_stage(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double tlim) {
-    if (d.fwx)
-        launch_stage_f<STAGE, true>(g, d, ntiles, mode, tlim);
-    else
-        launch_stage_f<STAGE, false>(g, d, ntiles, mode, tlim);
+    const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
+    switch (feat_of(d)) {
+    case 0: launch_fused_stage<0>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf); break;
+    case FEAT_FORCING:
+        launch_fused_stage<FEAT_FORCING>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf);
+        break;
+    case FEAT_SPONGE:
+        launch_fused_stage<FEAT_SPONGE>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf);
+        break;
+    default:
+        launch_fused_stage<FEAT_FORCING | FEAT_SPONGE>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf);
+        break;
+    }
 }
 
 // Shu-Osher table of a generic scheme, make_timestepper
@@ -128,24 +130,20 @@ SchemeTab scheme_tab(int scheme) {
     return t;
 }
 
-template <bool FRC>
-void launch_gen_f(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, int mode, double tlim) {
-    const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
-    if (hllc && nf)
-        hipLaunchKernelGGL((k_stage_gen<true, true, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
-    else if (hllc)
-        hipLaunchKernelGGL((k_stage_gen<true, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
-    else if (nf)
-        hipLaunchKernelGGL((k_stage_gen<false, true, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
-    else
-        hipLaunchKernelGGL((k_stage_gen<false, false, FRC>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, gs, g->parity, mode, tlim);
-}
-
 void launch_gen(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, int mode, double tlim) {
-    if (d.fwx)
-        launch_gen_f<true>(g, d, gs, ntiles, mode, tlim);
-    else
-        launch_gen_f<false>(g, d, gs, ntiles, mode, tlim);
+    const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
+    switch (feat_of(d)) {
+    case 0: launch_generic_stage<0>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf); break;
+    case FEAT_FORCING:
+        launch_generic_stage<FEAT_FORCING>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf);
+        break;
+    case FEAT_SPONGE:
+        launch_generic_stage<FEAT_SPONGE>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf);
+        break;
+    default:
+        launch_generic_stage<FEAT_FORCING | FEAT_SPONGE>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf);
+        break;
+    }
 }
 
 // Generic explicit SSP steps (euler / rk3 / rk43), solver_advance_2d_generic
@@ -155,6 +153,7 @@ void launch_gen(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, 
 int launch_generic(rkg_grid* g, int n, int mode, double tlim) {
     GridDesc d = make_desc(g);
     d.mpart = nullptr;  // the generic path keeps no mass budget (as the reference)
+    d.spart = nullptr;
     const int ntiles = d.nstrips * d.nsegs;
     const SchemeTab T = scheme_tab(g->scheme);
     double* A = plane_ptr(g, 0);
@@ -212,7 +211,7 @@ int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
         launch_stage<2>(g, d, ntiles, mode, tlim);
         if (d.mpart)
             hipLaunchKernelGGL(k_budget, dim3(1), dim3(256), 0, g->stream, g->ctrl, g->parity, d.mpart, ntiles, area,
-                               rain);
+                               rain, d.spart);
         g->parity ^= 1;
         if (g->prof) {
             float a = 0.f, b = 0.f;
@@ -492,9 +491,10 @@ int rkg_get_time_scheme(rkg_grid* g, int* scheme) {
 int rkg_set_mass_budget(rkg_grid* g, int enable) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     if (enable && !g->mpart) {
-        // room for the tiles of both stages at any tile height
+        // room for the tiles of both stages at any tile height, for the
+        // flux and the sponge sums
         const long long cap = (long long)((g->cfg.nx + WAVE - 1) / WAVE) * g->cfg.ny;
-        HIP_TRY(hipMalloc((void**)&g->mpart, 2 * cap * sizeof(double)));
+        HIP_TRY(hipMalloc((void**)&g->mpart, 4 * cap * sizeof(double)));
         g->mpart_cap = cap;
         CtrlSlot c;
         int rc = read_clock(g, &c);
@@ -526,6 +526,27 @@ int rkg_set_forcing(rkg_grid* g, int active) {
         HIP_TRY(hipStreamSynchronize(g->stream));
     }
     g->forcing = active != 0;
+    return RKG_OK;
+}
+
+int rkg_set_sponge(rkg_grid* g, int width, double strength, double h_ref) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (width < 0 || !std::isfinite(strength) || !std::isfinite(h_ref)) {
+        set_error("sponge width must be >= 0 and strength, h_ref finite");
+        return RKG_ERR_RANGE;
+    }
+    g->sponge_width = width;
+    g->sponge_strength = strength;
+    g->sponge_h_ref = h_ref;
+    return RKG_OK;
+}
+
+int rkg_get_sponge(rkg_grid* g, int* width, double* strength, double* h_ref) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!width || !strength || !h_ref) return RKG_ERR_RANGE;
+    *width = g->sponge_width;
+    *strength = g->sponge_strength;
+    *h_ref = g->sponge_h_ref;
     return RKG_OK;
 }
 

@@ -29,11 +29,14 @@ struct rkg_grid {
     double* save;        // 3 planes: U^n of the last step (RK save), or null
     double* nfield;      // per-cell Manning n (1 plane), or null
     double* cbuf;        // 3 planes: third stage buffer of rk3 / rk43, or null
-    double* mpart;       // per-tile flux_h sums of both stages (mass budget), or null
+    double* mpart;       // per-tile sums of both stages (mass budget): flux_h, then the sponge's
+                         // change of h (2 x mpart_cap each), or null
     long long mpart_cap; // tiles per stage that mpart holds
     int scheme;          // RKG_TIME_*
     double* frc;         // 4 planes: wind stress x, y, atmospheric pressure, precipitation, or null
     bool forcing;        // forcing%active: the stage kernels apply the planes in frc
+    int sponge_width;    // sponge relaxation next to RKG_BC_SPONGE edges (bc_params_t)
+    double sponge_strength, sponge_h_ref;
     rk::CtrlSlot* ctrl;      // device [2]
     rk::CtrlSlot* ctrl_host; // pinned [2]
     double* mass_buf;    // device [MASS_BLOCKS + 2]: partial sums, total mass, precipitation sum
@@ -82,6 +85,14 @@ inline int tile_rows_of(const rkg_grid* g) {
     return strips * ((g->cfg.ny + 15) / 16) >= 8192 ? 16 : 8;
 }
 
+// apply_sponge does something (rki_kernel_sponge.F90:42-47 and a sponge edge).
+inline bool sponge_on(const rkg_grid* g) {
+    if (g->sponge_width <= 0 || !(g->sponge_strength > 0.0)) return false;
+    for (int s = 0; s < 4; ++s)
+        if (g->cfg.bc[s] == RKG_BC_SPONGE) return true;
+    return false;
+}
+
 inline GridDesc make_desc(rkg_grid* g) {
     GridDesc d;
     memset(&d, 0, sizeof d);
@@ -115,6 +126,12 @@ inline GridDesc make_desc(rkg_grid* g) {
     d.nsegs = (c.ny + d.ty - 1) / d.ty;
     d.ctrl = g->ctrl;
     d.mpart = g->mpart;
+    if (sponge_on(g)) {
+        d.sponge_width = g->sponge_width;
+        d.sponge_strength = g->sponge_strength;
+        d.sponge_h_ref = g->sponge_h_ref;
+        d.spart = g->mpart ? g->mpart + 2 * g->mpart_cap : nullptr;
+    }
     if (g->forcing) {
         d.fwx = g->frc;
         d.fwy = g->frc + g->plane;

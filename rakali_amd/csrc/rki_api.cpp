@@ -581,7 +581,6 @@ bool supported(const Config& c, std::string& msg) {
     if (c.use_linear_bottom_drag) { msg = "use_linear_bottom_drag is not implemented here"; return false; }
     for (int s = 0; s < 4; ++s) {
         if (c.bc[s] == "nested" || c.bc[s] == "chapman") { msg = "boundary '" + c.bc[s] + "' is not implemented here"; return false; }
-        if (c.bc[s] == "sponge" && c.sponge_width > 0) { msg = "sponge relaxation (sponge_width > 0) is not implemented here"; return false; }
     }
     if (c.nghost < 2) { msg = "nghost must be >= 2 (the stencil reaches two cells)"; return false; }
     if (c.nx < 2 || c.ny < 2) { msg = "nx and ny must be >= 2 on this path"; return false; }
@@ -1143,6 +1142,9 @@ int rki_finalize(void* handle) {
     if (!rc) rc = rkg_rc(rkg_set_clock(s->dev, s->t_current, 0), "rkg_set_clock");
     if (!rc) rc = upload_all(*s);
     if (!rc) rc = push_forcing(*s);
+    if (!rc)   // bc_params_t sponge fields (rki_boundary_types.F90:147-149)
+        rc = rkg_rc(rkg_set_sponge(s->dev, s->cfg.sponge_width, s->cfg.sponge_strength, s->cfg.sponge_h_ref),
+                    "rkg_set_sponge");
     if (rc) {
         if (s->dev) rkg_destroy(&s->dev);
         return rc;

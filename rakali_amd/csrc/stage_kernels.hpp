@@ -61,7 +61,7 @@ __device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, i
 // (strip, segment, leader); the leader is the single tile per grid and
 // launch that advances the control slot.  STAGE 1 and 2 are the fused
 // SSP-RK2 stages; STAGE 3 is one stage `gs` of a generic SSP scheme.
-template <int STAGE, bool HLLC, bool NF, bool REC, class Where, bool FRC = false>
+template <int STAGE, bool HLLC, bool NF, bool REC, class Where, int FEAT = 0>
 __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p, int mode, double tlim,
                                            Where where, const GenStage& gs = GenStage{}) {
     const int3 w = where(d);
@@ -127,11 +127,11 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     s.rdx = recip(d.dx);
     s.rdy = recip(d.dy);
     const int strip = w.x, seg = w.y;
-    double smax, fsum;
+    double smax, fsum, ssum;
     if (tile_is_interior(d, strip, seg))
-        smax = process_tile<STAGE, HLLC, false, NF, REC, FRC>(d, s, L, strip, seg, fsum);
+        smax = process_tile<STAGE, HLLC, false, NF, REC, FEAT>(d, s, L, strip, seg, fsum, ssum);
     else
-        smax = process_tile<STAGE, HLLC, true, NF, REC, FRC>(d, s, L, strip, seg, fsum);
+        smax = process_tile<STAGE, HLLC, true, NF, REC, FEAT>(d, s, L, strip, seg, fsum, ssum);
     if (STAGE == 2 || STAGE == 3) {
         smax = wave_max(smax);
         if (threadIdx.x == 0 && smax > 0.0)
@@ -141,6 +141,11 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
         fsum = wave_sum(fsum);
         if (threadIdx.x == 0)
             d.mpart[(long long)(STAGE - 1) * d.nstrips * d.nsegs + (long long)seg * d.nstrips + strip] = fsum;
+    }
+    if ((FEAT & FEAT_SPONGE) && STAGE != 3 && d.spart != nullptr) {  // accumulate_sponge_mass_src
+        ssum = wave_sum(ssum);
+        if (threadIdx.x == 0)
+            d.spart[(long long)(STAGE - 1) * d.nstrips * d.nsegs + (long long)seg * d.nstrips + strip] = ssum;
     }
 }
 
@@ -152,18 +157,18 @@ struct SingleGridTile {
     }
 };
 
-// FRC: with the surface forcing (forcing_active).
-template <int STAGE, bool HLLC, bool NF, bool REC, bool FRC = false>
+// FEAT: FEAT_FORCING / FEAT_SPONGE source terms.
+template <int STAGE, bool HLLC, bool NF, bool REC, int FEAT = 0>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage(GridDesc d, int p, int mode, double tlim) {
     __shared__ TileLds L;
-    stage_tile<STAGE, HLLC, NF, REC, SingleGridTile, FRC>(d, L, p, mode, tlim, SingleGridTile{});
+    stage_tile<STAGE, HLLC, NF, REC, SingleGridTile, FEAT>(d, L, p, mode, tlim, SingleGridTile{});
 }
 
 // One stage of a generic SSP scheme, single grid.
-template <bool HLLC, bool NF, bool FRC = false>
+template <bool HLLC, bool NF, int FEAT = 0>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage_gen(GridDesc d, GenStage gs, int p, int mode, double tlim) {
     __shared__ TileLds L;
-    stage_tile<3, HLLC, NF, false, SingleGridTile, FRC>(d, L, p, mode, tlim, SingleGridTile{}, gs);
+    stage_tile<3, HLLC, NF, false, SingleGridTile, FEAT>(d, L, p, mode, tlim, SingleGridTile{}, gs);
 }
 
 // Resets control slot p to time t / step count n with an empty speed max;
@@ -184,28 +189,31 @@ __global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long lo
 // Boundary mass budget of one SSP-RK2 step (rki_solver.F90:507-537, called
 // after each stage's flux at :769 and :796): mass_out -= 0.5 dt sum(flux_h)
 // area per stage.  The per-tile sums of both stages are added in a fixed
-// order (strided per thread, then a tree).  With the surface forcing on,
-// `rain` holds the interior sum of the precipitation rate, and mass_src +=
-// 0.5 dt rain area per stage (accumulate_mass_src, rki_solver.F90:539-606,
-// called after each fused stage at :783 and :810).  Slot 1-p is the step's
-// result.
+// order (strided per thread, then a tree).  mass_src, per stage and in
+// the reference's order: with the surface forcing on, `rain` holds the
+// interior sum of the precipitation rate and mass_src += 0.5 dt rain area
+// (accumulate_mass_src, rki_solver.F90:539-606, at :783 and :810); with the
+// sponge on, `spart` holds the per-tile sums of its change of h and
+// mass_src += 0.5 sum area (accumulate_sponge_mass_src, :608-657, at :787
+// and :814).  Slot 1-p is the step's result.
 __global__ __launch_bounds__(256) void k_budget(CtrlSlot* ctrl, int p, const double* mpart, int ntiles,
-                                                double area, const double* rain) {
-    __shared__ double red[2][256];
+                                                double area, const double* rain, const double* spart) {
+    __shared__ double red[4][256];
     if (!ctrl[1 - p].active) return;
-    double a1 = 0.0, a2 = 0.0;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < ntiles; i += 256) {
-        a1 = a1 + mpart[i];
-        a2 = a2 + mpart[ntiles + i];
+        a[0] = a[0] + mpart[i];
+        a[1] = a[1] + mpart[ntiles + i];
+        if (spart != nullptr) {
+            a[2] = a[2] + spart[i];
+            a[3] = a[3] + spart[ntiles + i];
+        }
     }
-    red[0][threadIdx.x] = a1;
-    red[1][threadIdx.x] = a2;
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = a[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[0][threadIdx.x] = red[0][threadIdx.x] + red[0][threadIdx.x + s];
-            red[1][threadIdx.x] = red[1][threadIdx.x] + red[1][threadIdx.x + s];
-        }
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -215,13 +223,12 @@ __global__ __launch_bounds__(256) void k_budget(CtrlSlot* ctrl, int p, const dou
         m = m - 0.5 * dt * red[0][0] * area;
         m = m - 0.5 * dt * red[1][0] * area;
         q->mass_out = m;
-        if (rain != nullptr) {
-            const double r = *rain;
-            double src = q->mass_src;
-            src = src + 0.5 * dt * r * area;
-            src = src + 0.5 * dt * r * area;
-            q->mass_src = src;
+        double src = q->mass_src;
+        for (int st = 0; st < 2; ++st) {
+            if (rain != nullptr) src = src + 0.5 * dt * (*rain) * area;
+            if (spart != nullptr) src = src + 0.5 * red[2 + st][0] * area;
         }
+        q->mass_src = src;
     }
 }
 

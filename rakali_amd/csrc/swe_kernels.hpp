@@ -31,6 +31,10 @@ struct alignas(64) CtrlSlot {
     double mass_src;              // surface mass source (accumulate_mass_src, rki_solver.F90:539-606)
 };
 
+// Optional source terms of the stage kernels (template bit mask).
+constexpr int FEAT_FORCING = 1;   // surface forcing, rki_kernel_fused.F90:268-289
+constexpr int FEAT_SPONGE = 2;    // sponge relaxation, rki_kernel_sponge.F90:16-88
+
 struct GridDesc {
     int nx, ny;
     long long pitch;              // elements per row
@@ -44,8 +48,11 @@ struct GridDesc {
     double* rec;                  // 3 planes receiving each cell's flux divergence, or null
     double* mpart;                // per-tile sums of flux_h, [stage 1 tiles | stage 2 tiles], or null
     double* hist;                 // recording stage 1: (t, dt, dt recomputed) of the step, or null
-    const double* fwx; const double* fwy;   // surface forcing planes (FRC kernels): wind stress,
-    const double* fpres; const double* frain; // atmospheric pressure, precipitation
+    const double* fwx; const double* fwy;   // surface forcing planes (FEAT_FORCING kernels): wind
+    const double* fpres; const double* frain; // stress, atmospheric pressure, precipitation
+    double* spart;                // per-tile sums of the sponge's h change, [stage 1 | stage 2], or null
+    int sponge_width;             // sponge relaxation band (FEAT_SPONGE kernels), bc_params_t
+    double sponge_strength, sponge_h_ref;
     long long plane;              // elements per plane (pitch * ny)
     double ssp_coeff;             // stable-dt multiplier of the time scheme (rki_timestepper.F90)
     double g, cfl, manning_n, coriolis_f;
@@ -276,9 +283,35 @@ struct ExactTag {
 // shared-reciprocal ones; both give the IEEE quotients.
 // STAGE 1 writes U* to B; STAGE 2 reads U* and U^n, applies the RK2 average
 // and writes U^{n+1} over A, returning the lane's max CFL speed.
-template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC, bool FRC = false>
+// apply_sponge (rki_kernel_sponge.F90:16-88) on one interior cell (c, j),
+// 0-based: the distance to the nearest BC_SPONGE edge closer than the
+// width sets alpha = strength ((width - d) / width)^2.  dsum collects the
+// change of h.
+__device__ __forceinline__ void sponge_cell(const GridDesc& d, double dt, int c, int j, double& h,
+                                            double& hu, double& hv, double& dsum) {
+    const int sw = d.sponge_width;
+    int dmin = sw + 1;
+    const int dw = c, de = d.nx - 1 - c, ds = j, dn = d.ny - 1 - j;
+    if (d.bc[0] == BC_SPONGE && dw < sw && dw < dmin) dmin = dw;
+    if (d.bc[1] == BC_SPONGE && de < sw && de < dmin) dmin = de;
+    if (d.bc[2] == BC_SPONGE && ds < sw && ds < dmin) dmin = ds;
+    if (d.bc[3] == BC_SPONGE && dn < sw && dn < dmin) dmin = dn;
+    if (dmin < sw) {
+        const double ratio = (double)(sw - dmin) / (double)sw;
+        const double alpha = d.sponge_strength * ratio * ratio;
+        const double h_in = h;
+        h = h - dt * alpha * (h - d.sponge_h_ref);
+        hu = hu - dt * alpha * hu;
+        hv = hv - dt * alpha * hv;
+        dsum = dsum + (h - h_in);
+    }
+}
+
+template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC, int FEAT = 0>
 __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx& s, TileLds& L,
-                                               int strip, int seg, double& fsum) {
+                                               int strip, int seg, double& fsum, double& ssum) {
+    constexpr bool FRC = (FEAT & FEAT_FORCING) != 0;
+    constexpr bool SPG = (FEAT & FEAT_SPONGE) != 0;
     const int lane = threadIdx.x;
     const int c0 = strip * WAVE;
     const int c = c0 + lane;
@@ -289,6 +322,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     const long long pitch = d.pitch;
     double smax = 0.0;
     double facc = 0.0;  // sum of the lane's flux_h, rows in order
+    double sacc = 0.0;  // sum of the lane's sponge change of h
     const bool read_un = STAGE == 2 || (STAGE == 3 && s.blend);
 
     // (1) edge columns of the segment, one row per lane
@@ -409,12 +443,15 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                     stage_body<EX, true, FRC>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, d.nfield[k], fr);
                 else
                     stage_body<EX, false, FRC>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, 0.0, fr);
+                // the sponge follows each fused stage (rki_solver.F90:787,814)
+                if (STAGE == 1 && SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);
                 if (STAGE == 2) {
                     // rki_kernel_fused.F90:435-463
                     h = 0.5 * (un.h + h);
                     hu = 0.5 * (un.hu + hu);
                     hv = 0.5 * (un.hv + hv);
                     clamp_desing<EX>(g, h, hu, hv);
+                    if (SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);
                     smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
                 }
                 if (STAGE == 3) {
@@ -424,6 +461,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                         hv = s.alpha * un.hv + s.a_prev * hv;
                         clamp_desing<EX>(g, h, hu, hv);
                     }
+                    if (SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);   // :919-920, at beta_i dt
                     if (s.last) smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
                 }
                 s.oh[k] = h; s.ohu[k] = hu; s.ohv[k] = hv;
@@ -453,6 +491,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         }
     }
     fsum = facc;
+    ssum = sacc;
     return smax;
 }
 

@@ -158,9 +158,24 @@ class Grid:
         check("rkg_set_forcing", self._lib.rkg_set_forcing(self.handle, int(bool(active))))
 
     @property
+    def sponge(self):
+        """(width, strength, h_ref) of the sponge relaxation next to "sponge"
+        edges (apply_sponge, rki_kernel_sponge.F90); width 0 is off."""
+        w, s, h = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check("rkg_get_sponge", self._lib.rkg_get_sponge(self.handle, ctypes.byref(w), ctypes.byref(s),
+                                                         ctypes.byref(h)))
+        return w.value, s.value, h.value
+
+    @sponge.setter
+    def sponge(self, value) -> None:
+        width, strength, h_ref = value
+        check("rkg_set_sponge", self._lib.rkg_set_sponge(self.handle, int(width), float(strength), float(h_ref)))
+
+    @property
     def mass_src(self) -> float:
-        """Volume added by precipitation since the mass budget was enabled
-        (accumulate_mass_src, rki_solver.F90:539-606)."""
+        """Volume added by precipitation and the sponge since the mass budget
+        was enabled (accumulate_mass_src and accumulate_sponge_mass_src,
+        rki_solver.F90:539-657)."""
         m = ctypes.c_double()
         check("rkg_get_mass_source", self._lib.rkg_get_mass_source(self.handle, ctypes.byref(m)))
         return m.value

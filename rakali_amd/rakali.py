@@ -11,9 +11,8 @@ garbage collection).  The library it binds is ``rakali_amd/librakali.so``
 fallback, and a missing library raises ``FileNotFoundError``.
 
 Differences, all from features this build does not run: multilayer and
-non-hydrostatic simulations, forcing fields, sponge relaxation, nested and
-Chapman edges and time schemes other than rk2 raise ``RuntimeError`` (the
-library returns -5); ``save_restart`` writes the library's binary restart
+non-hydrostatic simulations, nested and Chapman edges and the
+semi-implicit scheme raise ``RuntimeError`` (the library returns -5); ``save_restart`` writes the library's binary restart
 format instead of NetCDF; ``to_xarray``/``save_checkpoint`` need xarray.
 """
 from __future__ import annotations
