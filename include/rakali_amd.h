@@ -200,6 +200,32 @@ int rkg_get_forcing(rkg_grid *g, int *active);
  * per stage, and the sponge's change of volume (rkg_set_sponge).
  * rkg_set_mass_budget(g, 1) starts it at 0. */
 int rkg_get_mass_source(rkg_grid *g, double *mass_src);
+/* Gauge stations (rki_gauges.F90): points (x, y) in metres from the grid's
+ * lower-left corner.  rkg_gauges_set keeps the stations inside the domain
+ * (compute_local_gauges, :208-243; n_local of them, in input order,
+ * rkg_gauges_local gives their input indices) with their bilinear
+ * interpolation cells and weights (compute_interp_weights, :245-336: the
+ * two cell centres around the point, clamped to the edge cell in the
+ * outer half cells); n = 0 removes them.  A sample is eta = h + b and
+ * u, v = hu, hv / max(h, 1e-6), each interpolated in the reference's
+ * operation order (gauges_write, :490-575). */
+#define RKG_MAX_GAUGES 1048576
+int rkg_gauges_set(rkg_grid *g, int n, const double *x, const double *y, int *n_local);
+int rkg_gauges_local(rkg_grid *g, int *local_idx);
+/* One sample of the current state (n_local values each). */
+int rkg_gauges_sample(rkg_grid *g, double *eta, double *u, double *v);
+/* Records a sample after every step on the device, at the driver's cadence
+ * (rki_driver.F90:345-352,470-474): with interval > 0, when the step's end
+ * time reaches next = t_start + interval, next += interval; with
+ * interval <= 0, after every step.  `capacity` records fit in the device
+ * buffer, which is drained to host memory when it may fill (a pause of the
+ * step loop); capacity 0 stops recording.  Starting clears the series. */
+int rkg_gauges_record(rkg_grid *g, double interval, int capacity);
+/* Takes up to max_records of the oldest recorded samples: times t
+ * [n_records], eta, u, v [n_records][n_local] (station fastest), and
+ * removes them from the series.  With t == NULL, returns the count only. */
+int rkg_gauges_series(rkg_grid *g, long long max_records, long long *n_records, double *t, double *eta,
+                      double *u, double *v);
 /* Sponge relaxation (apply_sponge, rki_kernel_sponge.F90:16-88; namelist
  * sponge_width, sponge_strength, sponge_h_ref, bc_params_t
  * rki_boundary_types.F90:73-77).  Interior cells closer than `width`

@@ -91,6 +91,10 @@ def lib():
         L.orc_fused_stage2.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
         L.orc_apply_sponge.argtypes = [P, dp, dp, dp, ctypes.c_double]
         L.orc_fused_stage_generic.argtypes = [P] + [dp] * 6 + [ctypes.c_double]
+        ip = ctypes.c_void_p
+        L.orc_gauges_setup.argtypes = [P, ctypes.c_int] + [ip] * 7
+        L.orc_gauges_setup.restype = ctypes.c_int
+        L.orc_gauges_sample.argtypes = [P, dp, dp, dp, dp, ctypes.c_int] + [ip] * 7
         L.orc_compute_timestep.argtypes = [P, dp, dp, dp]
         L.orc_compute_timestep.restype = ctypes.c_double
         L.orc_tidal_eta.argtypes = [P, ctypes.c_double]
@@ -151,6 +155,37 @@ def fused_stage1(p, h, hu, hv, fh, fhu, fhv, dt):
 
 def apply_sponge(p, h, hu, hv, dt):
     lib().orc_apply_sponge(ctypes.byref(p), _ptr(h), _ptr(hu), _ptr(hv), dt)
+
+
+class Gauges:
+    """Stations at (x, y) metres: compute_local_gauges and
+    compute_interp_weights (rki_gauges.F90:208-336).  ix0 / iy0 are the
+    reference's 1-based ghosted array indices."""
+
+    def __init__(self, p, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        n = len(x)
+        self.local_idx = np.zeros(n, np.int32)
+        self.ix0 = np.zeros(n, np.int32)
+        self.iy0 = np.zeros(n, np.int32)
+        self.wx = np.zeros(n)
+        self.wy = np.zeros(n)
+        self.n_local = lib().orc_gauges_setup(ctypes.byref(p), n, x.ctypes.data, y.ctypes.data,
+                                              self.local_idx.ctypes.data, self.ix0.ctypes.data,
+                                              self.iy0.ctypes.data, self.wx.ctypes.data, self.wy.ctypes.data)
+        for k in ("local_idx", "ix0", "iy0", "wx", "wy"):
+            setattr(self, k, getattr(self, k)[:self.n_local].copy())
+        self.p = p
+
+    def sample(self, h, hu, hv, b):
+        """(eta, u, v) at the local stations: gauges_write (:516-550)."""
+        out = [np.zeros(self.n_local) for _ in range(3)]
+        lib().orc_gauges_sample(ctypes.byref(self.p), _ptr(h), _ptr(hu), _ptr(hv), _ptr(b), self.n_local,
+                                self.ix0.ctypes.data, self.iy0.ctypes.data, self.wx.ctypes.data,
+                                self.wy.ctypes.data, out[0].ctypes.data, out[1].ctypes.data,
+                                out[2].ctypes.data)
+        return tuple(out)
 
 
 def fused_stage_generic(p, h, hu, hv, fh, fhu, fhv, dt):

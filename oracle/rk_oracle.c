@@ -577,6 +577,79 @@ void orc_ssp_combine(const orc_params *p, double *h, double *hu, double *hv,
     }
 }
 
+/* ---------------------------------------------------------------------- */
+/* Gauges (rki_gauges.F90)                                                */
+/* ---------------------------------------------------------------------- */
+
+/* floor(v) + 1 as the reference's integer cell index, or a value outside
+ * [1, n] when v lies off the grid (kept finite for the cast). */
+static int gauge_cell(double v, double d, int n)
+{
+    const double f = floor(v / d);
+    if (!(f >= -1.0) || f > (double)n) return -1;
+    return (int)f + 1;
+}
+
+int orc_gauges_setup(const orc_params *p, int n, const double *x, const double *y,
+                     int *local_idx, int *ix0, int *iy0, double *wx, double *wy)
+{
+    const double dx = p->dx, dy = p->dy;
+    const int ng = p->ng, nx = p->nx, ny = p->ny;
+    int nl = 0;
+    for (int g = 0; g < n; ++g) {   /* compute_local_gauges :208-243 */
+        const int ig = gauge_cell(x[g], dx, nx), jg = gauge_cell(y[g], dy, ny);
+        if (ig >= 1 && ig <= nx && jg >= 1 && jg <= ny) local_idx[nl++] = g;
+    }
+    for (int l = 0; l < nl; ++l) {   /* compute_interp_weights :245-336 */
+        const double gx = x[local_idx[l]], gy = y[local_idx[l]];
+        const int ig = gauge_cell(gx, dx, nx), jg = gauge_cell(gy, dy, ny);
+        const int ia = ig + ng, ja = jg + ng;
+        const double xc = ((double)ig - 0.5) * dx, yc = ((double)jg - 0.5) * dy;
+        double w;
+        if (gx < xc) {
+            if (ig > 1) { ix0[l] = ia - 1; w = (gx - (xc - dx)) / dx; }
+            else { ix0[l] = ia; w = 0.0; }
+        } else {
+            if (ig < nx) { ix0[l] = ia; w = (gx - xc) / dx; }
+            else { ix0[l] = ia; w = 0.0; }
+        }
+        wx[l] = fmax(0.0, fmin(1.0, w));
+        if (gy < yc) {
+            if (jg > 1) { iy0[l] = ja - 1; w = (gy - (yc - dy)) / dy; }
+            else { iy0[l] = ja; w = 0.0; }
+        } else {
+            if (jg < ny) { iy0[l] = ja; w = (gy - yc) / dy; }
+            else { iy0[l] = ja; w = 0.0; }
+        }
+        wy[l] = fmax(0.0, fmin(1.0, w));
+    }
+    return nl;
+}
+
+static double bilinear(const orc_params *p, const double *f, int i0, int j0, double wx, double wy)
+{
+    const int nxt = NT(p);
+    return (1.0 - wx) * (1.0 - wy) * f[IX(i0, j0)] + wx * (1.0 - wy) * f[IX(i0 + 1, j0)]
+           + (1.0 - wx) * wy * f[IX(i0, j0 + 1)] + wx * wy * f[IX(i0 + 1, j0 + 1)];
+}
+
+void orc_gauges_sample(const orc_params *p, const double *h, const double *hu,
+                       const double *hv, const double *b, int n_local, const int *ix0,
+                       const int *iy0, const double *wx, const double *wy,
+                       double *eta, double *u, double *v)
+{
+    for (int g = 0; g < n_local; ++g) {
+        const double hi = bilinear(p, h, ix0[g], iy0[g], wx[g], wy[g]);
+        const double hui = bilinear(p, hu, ix0[g], iy0[g], wx[g], wy[g]);
+        const double hvi = bilinear(p, hv, ix0[g], iy0[g], wx[g], wy[g]);
+        const double bi = bilinear(p, b, ix0[g], iy0[g], wx[g], wy[g]);
+        eta[g] = hi + bi;
+        const double hs = fmax(hi, 1.0e-6);
+        u[g] = hui / hs;
+        v[g] = hvi / hs;
+    }
+}
+
 /* apply_sponge, rki_kernel_sponge.F90:16-88: interior cells closer than
  * sponge_width cells to a BC_SPONGE edge relax toward (h_ref, 0, 0) with
  * alpha = strength ((width - d) / width)^2, d the distance in cells from

@@ -90,6 +90,19 @@ void orc_ssp_combine(const orc_params *p, double *h, double *hu, double *hv,
                      double a_n);
 /* apply_sponge (rki_kernel_sponge.F90:16-88). */
 void orc_apply_sponge(const orc_params *p, double *h, double *hu, double *hv, double dt);
+/* Gauges (rki_gauges.F90): compute_local_gauges + compute_interp_weights
+ * (:208-336) for n stations at (x, y) in metres.  Returns n_local; writes
+ * local_idx (0-based station ids inside the domain), ix0 / iy0 (1-based
+ * ghosted array indices of the lower-left interpolation cell, as the
+ * reference) and the weights wx, wy. */
+int orc_gauges_setup(const orc_params *p, int n, const double *x, const double *y,
+                     int *local_idx, int *ix0, int *iy0, double *wx, double *wy);
+/* gauges_write's bilinear interpolation (:516-550): eta = h + b and the
+ * depth-averaged velocities with the 1e-6 m depth guard. */
+void orc_gauges_sample(const orc_params *p, const double *h, const double *hu,
+                       const double *hv, const double *b, int n_local, const int *ix0,
+                       const int *iy0, const double *wx, const double *wy,
+                       double *eta, double *u, double *v);
 double orc_compute_timestep(const orc_params *p, const double *h,
                             const double *hu, const double *hv);
 double orc_tidal_eta(const orc_params *p, double time);

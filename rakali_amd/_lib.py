@@ -98,6 +98,11 @@ SIGNATURES = {
     "rkg_get_forcing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     "rkg_get_mass_source": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
     "rkg_set_sponge": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "rkg_gauges_set": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.POINTER(ctypes.c_int)]),
+    "rkg_gauges_local": (ctypes.c_int, [_P, _P]),
+    "rkg_gauges_sample": (ctypes.c_int, [_P, _P, _P, _P]),
+    "rkg_gauges_record": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int]),
+    "rkg_gauges_series": (ctypes.c_int, [_P, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong), _P, _P, _P, _P]),
     "rkg_get_sponge": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]),
     "rkg_get_config": (ctypes.c_int, [_P, ctypes.POINTER(Config)]),

@@ -41,7 +41,7 @@ void rk::set_error(const char* fmt, ...) {
 
 namespace {
 
-constexpr unsigned GRID_MAGIC = 0x72616B47u;  // 'rakG'
+constexpr unsigned GRID_MAGIC = GRID_MAGIC_LIVE;
 constexpr unsigned GRID_DEAD = 0xDEADBEEFu;
 
 // Plain copy, 16 bytes per lane per iteration: the bandwidth reference
@@ -179,12 +179,41 @@ int launch_generic(rkg_grid* g, int n, int mode, double tlim) {
             hipLaunchKernelGGL(k_euler_finish, dim3(blocks), dim3(256), 0, g->stream, d, g->parity);
         }
         g->parity ^= 1;
+        if (g->gauges) {
+            int rc = gauges_after_step(g, g->parity);
+            if (rc) return rc;
+        }
     }
     HIP_TRY(hipGetLastError());
     return RKG_OK;
 }
 
+int launch_steps_now(rkg_grid* g, int n, int mode, double tlim);
+
+// Steps in pieces that fit the gauge series' device buffer, drained between
+// pieces (each step records at most one gauge record).  n = 0 still makes
+// the control slot's CFL speed current (callers rely on rkg_step(g, 0)).
 int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
+    if (g->speed_stale) {
+        int rc = refresh_speed(g);
+        if (rc) return rc;
+    }
+    while (n > 0) {
+        long long room = gauges_room(g);
+        if (room == 0) {
+            int rc = gauges_drain(g);
+            if (rc) return rc;
+            room = gauges_room(g);
+        }
+        const int piece = (int)std::min<long long>(n, room);
+        int rc = launch_steps_now(g, piece, mode, tlim);
+        if (rc) return rc;
+        n -= piece;
+    }
+    return RKG_OK;
+}
+
+int launch_steps_now(rkg_grid* g, int n, int mode, double tlim) {
     if (g->speed_stale) {
         int rc = refresh_speed(g);
         if (rc) return rc;
@@ -213,6 +242,10 @@ int launch_steps(rkg_grid* g, int n, int mode, double tlim) {
             hipLaunchKernelGGL(k_budget, dim3(1), dim3(256), 0, g->stream, g->ctrl, g->parity, d.mpart, ntiles, area,
                                rain, d.spart);
         g->parity ^= 1;
+        if (g->gauges) {
+            int rc = gauges_after_step(g, g->parity);
+            if (rc) return rc;
+        }
         if (g->prof) {
             float a = 0.f, b = 0.f;
             HIP_TRY(hipEventRecord(g->ev[2], g->stream));
@@ -332,6 +365,7 @@ void rkg_destroy(rkg_grid** pg) {
     if (g->cbuf) (void)hipFree(g->cbuf);
     if (g->mpart) (void)hipFree(g->mpart);
     if (g->frc) (void)hipFree(g->frc);
+    gauges_free(g);
     if (g->ctrl) (void)hipFree(g->ctrl);
     if (g->mass_buf) (void)hipFree(g->mass_buf);
     if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);

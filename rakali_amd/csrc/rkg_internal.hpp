@@ -16,6 +16,10 @@ __attribute__((visibility("hidden"), format(printf, 1, 2))) void set_error(const
 
 }  // namespace rk
 
+namespace rk {
+struct GaugeState;   // gauge stations and their time series (rkg_gauges.hip)
+}
+
 // One grid: device buffers, stream and control slots (rkg_grid of the ABI).
 struct rkg_grid {
     unsigned magic;
@@ -48,11 +52,23 @@ struct rkg_grid {
     hipEvent_t ev[3];
     double prof_ms[2];     // accumulated stage-1 / stage-2 kernel time
     long long prof_n;
+    rk::GaugeState* gauges;  // stations, or null
 };
 
 namespace rk {
 
+constexpr unsigned GRID_MAGIC_LIVE = 0x72616B47u;  // 'rakG', rkg_grid::magic of a live grid
+inline bool grid_valid(const rkg_grid* g) { return g != nullptr && g->magic == GRID_MAGIC_LIVE; }
+
 inline double* plane_ptr(rkg_grid* g, int k) { return g->buf + (size_t)k * g->plane; }
+
+// Gauge recording around the step launches (rkg_gauges.hip): the steps one
+// launch may take before the device series must be drained, the drain, the
+// record after a step (slot p holds the state after it), and the release.
+long long gauges_room(const rkg_grid* g);
+int gauges_drain(rkg_grid* g);
+int gauges_after_step(rkg_grid* g, int p);
+void gauges_free(rkg_grid* g);
 
 inline double* field_ptr(rkg_grid* g, int field) {
     switch (field) {

@@ -180,6 +180,50 @@ class Grid:
         check("rkg_get_mass_source", self._lib.rkg_get_mass_source(self.handle, ctypes.byref(m)))
         return m.value
 
+    def set_gauges(self, x, y) -> np.ndarray:
+        """Gauge stations at (x, y) metres (rki_gauges.F90); returns the
+        indices of those inside the domain, whose samples the other gauge
+        calls return in that order."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+        if x.shape != y.shape:
+            raise ValueError("x and y must have the same length")
+        nl = ctypes.c_int()
+        check("rkg_gauges_set", self._lib.rkg_gauges_set(self.handle, len(x), x.ctypes.data, y.ctypes.data,
+                                                         ctypes.byref(nl)))
+        idx = np.zeros(nl.value, np.int32)
+        if nl.value:
+            check("rkg_gauges_local", self._lib.rkg_gauges_local(self.handle, idx.ctypes.data))
+        self._n_gauges = nl.value
+        return idx
+
+    def sample_gauges(self):
+        """(eta, u, v) at the stations now."""
+        n = getattr(self, "_n_gauges", 0)
+        out = [np.zeros(n) for _ in range(3)]
+        check("rkg_gauges_sample", self._lib.rkg_gauges_sample(self.handle, *[a.ctypes.data for a in out]))
+        return tuple(out)
+
+    def record_gauges(self, interval: float = 0.0, capacity: int = 4096) -> None:
+        """Record a sample after each step at the driver's cadence (every
+        `interval` seconds of model time, or every step for interval <= 0);
+        capacity 0 stops.  Clears the series."""
+        check("rkg_gauges_record", self._lib.rkg_gauges_record(self.handle, float(interval), int(capacity)))
+
+    def gauge_series(self):
+        """The recorded samples since the last call: dict of t [records] and
+        eta, u, v [records, stations]."""
+        n = getattr(self, "_n_gauges", 0)
+        cnt = ctypes.c_longlong()
+        check("rkg_gauges_series", self._lib.rkg_gauges_series(self.handle, 0, ctypes.byref(cnt), None, None,
+                                                               None, None))
+        k = cnt.value
+        t = np.zeros(k)
+        eta, u, v = (np.zeros((k, n)) for _ in range(3))
+        check("rkg_gauges_series", self._lib.rkg_gauges_series(
+            self.handle, k, ctypes.byref(cnt), t.ctypes.data, eta.ctypes.data, u.ctypes.data, v.ctypes.data))
+        return {"t": t, "eta": eta, "u": u, "v": v}
+
     def device_ptr(self, name: str):
         p = ctypes.c_void_p()
         pitch = ctypes.c_longlong()
