@@ -223,7 +223,8 @@ int rkg_gauges_sample(rkg_grid *g, double *eta, double *u, double *v);
 int rkg_gauges_record(rkg_grid *g, double interval, int capacity);
 /* Takes up to max_records of the oldest recorded samples: times t
  * [n_records], eta, u, v [n_records][n_local] (station fastest), and
- * removes them from the series.  With t == NULL, returns the count only. */
+ * removes them from the series.  With t == NULL or max_records == 0,
+ * returns the count only. */
 int rkg_gauges_series(rkg_grid *g, long long max_records, long long *n_records, double *t, double *eta,
                       double *u, double *v);
 /* Sponge relaxation (apply_sponge, rki_kernel_sponge.F90:16-88; namelist

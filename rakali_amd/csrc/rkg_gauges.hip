@@ -312,7 +312,7 @@ int rkg_gauges_series(rkg_grid* g, long long max_records, long long* n_records, 
     int rc = gauges_drain(g);
     if (rc) return rc;
     const long long have = s->host_records;
-    if (!t) { *n_records = have; return RKG_OK; }
+    if (!t || max_records == 0) { *n_records = have; return RKG_OK; }
     if (!eta || !u || !v || max_records < 0) return RKG_ERR_RANGE;
     const long long k = std::min(have, max_records);
     const size_t n = (size_t)s->n_local, rec = 1 + 3 * n;

@@ -17,6 +17,7 @@ module rakali_amd_api
    public :: rki_get_h_ptr, rki_get_hu_ptr, rki_get_hv_ptr, rki_get_b_ptr
    public :: rki_set_bathymetry, rki_set_initial_h, rki_sync_host, rki_last_error
    public :: rki_get_device_grid, rkg_total_mass
+   public :: rkg_gauges_set, rkg_gauges_local, rkg_gauges_record, rkg_gauges_series
 
    interface
       function rki_create_from_namelist(fname, flen, handle) result(ierr) &
@@ -163,6 +164,40 @@ module rakali_amd_api
          import :: c_ptr, c_int, c_double
          type(c_ptr), value, intent(in) :: grid
          real(c_double), intent(out) :: mass
+         integer(c_int) :: ierr
+      end function
+
+      function rkg_gauges_set(grid, n, x, y, n_local) result(ierr) bind(c, name="rkg_gauges_set")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value, intent(in) :: grid
+         integer(c_int), value, intent(in) :: n
+         real(c_double), intent(in) :: x(*), y(*)
+         integer(c_int), intent(out) :: n_local
+         integer(c_int) :: ierr
+      end function
+
+      function rkg_gauges_local(grid, local_idx) result(ierr) bind(c, name="rkg_gauges_local")
+         import :: c_ptr, c_int
+         type(c_ptr), value, intent(in) :: grid
+         integer(c_int), intent(out) :: local_idx(*)
+         integer(c_int) :: ierr
+      end function
+
+      function rkg_gauges_record(grid, interval, capacity) result(ierr) bind(c, name="rkg_gauges_record")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value, intent(in) :: grid
+         real(c_double), value, intent(in) :: interval
+         integer(c_int), value, intent(in) :: capacity
+         integer(c_int) :: ierr
+      end function
+
+      function rkg_gauges_series(grid, max_records, n_records, t, eta, u, v) result(ierr) &
+         bind(c, name="rkg_gauges_series")
+         import :: c_ptr, c_int, c_long_long, c_double
+         type(c_ptr), value, intent(in) :: grid
+         integer(c_long_long), value, intent(in) :: max_records
+         integer(c_long_long), intent(out) :: n_records
+         real(c_double), intent(out) :: t(*), eta(*), u(*), v(*)
          integer(c_int) :: ierr
       end function
    end interface

@@ -395,3 +395,43 @@ def test_fortran_driver_matches_python_class(tmp_path):
         for k in range(1, 5):
             sim.advance(20.0 * k / 4)
         assert np.array_equal(sim.h_phys, h_f)
+
+
+def test_fortran_driver_gauges(tmp_path):
+    """The driver's gauge output: the stations of gauge_file (comments,
+    blank and malformed lines skipped, stations outside dropped), records
+    at the gauge_output_interval cadence, and the last record equal to the
+    oracle's stations sampled on the final full arrays."""
+    import oracle as O
+    from rakali_amd import Rakali
+    drv = ROOT / "rakali_amd" / "rakali_driver"
+    if not drv.exists():
+        pytest.skip("Fortran driver not built (no Fortran compiler)")
+    gfile = tmp_path / "stations.txt"
+    gfile.write_text("# name x y\n\nmouth 12.5 8.3\n  mid 100.0 60.0\nbad line here\nfar 900.0 10.0\n"
+                     "edge 199.0 0.5\n")
+    cfg = dict(nx=40, ny=24, dx=5.0, dy=5.0, t_end=20.0, bc_west="tidal", tidal_amplitude=0.3,
+               tidal_period=60.0, init_condition="gaussian_hump", hump_sigma=20.0,
+               hump_amplitude=0.3, manning_n=0.02, use_hllc=True)
+    nml = tmp_path / "run.nml"
+    nml.write_text(Rakali._kwargs_to_namelist(cfg) + "\n&output_nml\n gauge_file = '" + str(gfile)
+                   + "'\n gauge_output_interval = 2.0\n output_dir = '" + str(tmp_path) + "'\n/\n")
+    res = subprocess.run([str(drv), str(nml), "--chunks", "4"], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stderr
+    assert "gauges: 3 of 4 inside the domain" in res.stdout
+    lines = (tmp_path / "gauges.txt").read_text().splitlines()
+    st = [ln.split() for ln in lines if ln.startswith("# station")]
+    assert [s[2] for s in st] == ["mouth", "mid", "edge"]
+    rows = np.array([[float(v) for v in ln.split()] for ln in lines if not ln.startswith("#")])
+    t = rows[:, 0]
+    assert np.all(np.diff(t) > 0) and t[-1] == 20.0
+    assert len(t) == 10 and np.all(t >= 2.0 * np.arange(1, 11) - 1e-9)
+    with Rakali(**cfg) as sim:
+        for k in range(1, 5):
+            sim.advance(20.0 * k / 4)
+        p = O.make_params(40, 24, ng=sim.h.shape[0] - 40 >> 1, dx=5.0, dy=5.0)
+        G = O.Gauges(p, [12.5, 100.0, 199.0], [8.3, 60.0, 0.5])
+        eta, u, v = G.sample(*(np.asfortranarray(a) for a in (sim.h, sim.hu, sim.hv, sim.b)))
+    last = rows[-1, 1:].reshape(3, 3)
+    assert np.array_equal(last[:, 0], eta) and np.array_equal(last[:, 1], u) and np.array_equal(last[:, 2], v)
