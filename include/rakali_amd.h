@@ -315,6 +315,12 @@ typedef struct rkg_adjoint rkg_adjoint;     /* opaque handle */
  * targets: n_steps x n_gauges, step-major (may be NULL: zeros). */
 int rkg_adjoint_create(rkg_grid *g, int n_steps, int checkpoint_interval, int n_gauges, const int *gauge_i,
                        const int *gauge_j, const double *targets, rkg_adjoint **out);
+/* The same loss over stations at (x[k], y[k]) metres, eta sampled as
+ * rkg_gauges_sample does (bilinear, rki_gauges.F90:245-336,516-550); every
+ * station must lie inside the domain.  No counterpart in the reference,
+ * whose gauge series is output only. */
+int rkg_adjoint_create_stations(rkg_grid *g, int n_steps, int checkpoint_interval, int n_stations,
+                                const double *x, const double *y, const double *targets, rkg_adjoint **out);
 void rkg_adjoint_destroy(rkg_adjoint **a);
 int rkg_adjoint_set_targets(rkg_adjoint *a, const double *targets);
 /* n_steps steps from the grid's current state; returns L; the grid is

@@ -122,6 +122,8 @@ SIGNATURES = {
     "rkg_ensemble_autotune": (ctypes.c_int, [_P, ctypes.c_int, _dp]),
     "rkg_adjoint_create": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P,
                                           ctypes.POINTER(_P)]),
+    "rkg_adjoint_create_stations": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P,
+                                                   ctypes.POINTER(_P)]),
     "rkg_adjoint_destroy": (None, [ctypes.POINTER(_P)]),
     "rkg_adjoint_set_targets": (ctypes.c_int, [_P, _P]),
     "rkg_adjoint_forward": (ctypes.c_int, [_P, _dp]),

@@ -4,7 +4,9 @@
 
     L = sum_{steps s} sum_{gauges k} (h + b - target_{s,k})^2,
 
-with eta = h + b read at the gauge cells after every step.  ``forward``
+with eta = h + b read at the gauge cells after every step, or, with
+``stations=(x, y)`` in metres, sampled bilinearly at those points as the
+gauge recorder does (rki_gauges.F90).  ``forward``
 runs the steps from the grid's current state, keeping a checkpoint every
 ``checkpoint_interval`` steps; ``backward`` replays each segment and
 reverses it, giving dL/db, dL/dn (per-cell Manning n) and dL/d(h, hu, hv)
@@ -29,14 +31,28 @@ from ._lib import FIELDS, check
 
 
 class GaugeMisfit:
-    def __init__(self, grid, n_steps: int, checkpoint_interval: int, gauges, targets=None):
+    def __init__(self, grid, n_steps: int, checkpoint_interval: int, gauges=None, targets=None, stations=None):
         self._L = _lib.load()
         self.grid = grid
+        self.n_steps = int(n_steps)
+        self._h = ctypes.c_void_p()
+        if stations is not None:
+            if gauges is not None:
+                raise ValueError("give gauge cells or stations, not both")
+            x = np.ascontiguousarray(stations[0], dtype=np.float64).ravel()
+            y = np.ascontiguousarray(stations[1], dtype=np.float64).ravel()
+            if x.shape != y.shape:
+                raise ValueError("station x and y must have the same length")
+            self.n_gauges = len(x)
+            t = None if targets is None else self._targets(targets)
+            check("rkg_adjoint_create_stations", self._L.rkg_adjoint_create_stations(
+                grid.handle, self.n_steps, int(checkpoint_interval), self.n_gauges, x.ctypes.data, y.ctypes.data,
+                None if t is None else t.ctypes.data_as(ctypes.c_void_p), ctypes.byref(self._h)))
+            return
         gi = np.ascontiguousarray([int(i) for i, _ in gauges], dtype=np.int32)
         gj = np.ascontiguousarray([int(j) for _, j in gauges], dtype=np.int32)
-        self.n_steps, self.n_gauges = int(n_steps), len(gi)
+        self.n_gauges = len(gi)
         t = None if targets is None else self._targets(targets)
-        self._h = ctypes.c_void_p()
         check("rkg_adjoint_create", self._L.rkg_adjoint_create(
             grid.handle, self.n_steps, int(checkpoint_interval), self.n_gauges,
             gi.ctypes.data_as(ctypes.c_void_p), gj.ctypes.data_as(ctypes.c_void_p),

@@ -31,6 +31,7 @@
 
 #include "../../include/rakali_amd.h"
 #include "adjoint.hpp"
+#include "gauge_eval.hpp"
 #include "rkg_internal.hpp"
 #include "stage_kernels.hpp"
 
@@ -481,17 +482,35 @@ __device__ double eta_part(const GridDesc& d, double eta, const double* in, cons
     return acc;
 }
 
-// Gauge misfit of one state: loss += sum_k (h + b - target_k)^2, eta
+// Gauges of a session: cells (eta = h + b of the cell) or bilinear
+// stations (gauge_eval.hpp) with lower-left cell (gi, gj) and weights gw
+// (wx then wy).
+struct GaugeSpec {
+    const int* gi;
+    const int* gj;
+    const double* gw;   // null: cell gauges
+    int ng, nx, ny;
+};
+
+__device__ __forceinline__ double spec_eta(const GaugeSpec& G, const double* h, const double* bed, long long pitch,
+                                           int k) {
+    if (G.gw == nullptr) {
+        const long long o = (long long)G.gj[k] * pitch + G.gi[k];
+        return h[o] + bed[o];
+    }
+    return station_eta(h, bed, pitch, G.nx, G.ny, G.gi[k], G.gj[k], G.gw[k], G.gw[G.ng + k]);
+}
+
+// Gauge misfit of one state: loss += sum_k (eta_k - target_k)^2, eta
 // recorded per gauge; a single workgroup, fixed summation order.
 __global__ __launch_bounds__(256) void k_gauge_loss(const double* __restrict__ h, const double* __restrict__ bed,
-                                                   long long pitch, const int* __restrict__ gi,
-                                                   const int* __restrict__ gj, const double* __restrict__ tgt,
-                                                   int ng, double* __restrict__ eta_out, double* __restrict__ loss) {
+                                                   long long pitch, GaugeSpec G, const double* __restrict__ tgt,
+                                                   double* __restrict__ eta_out, double* __restrict__ loss) {
     __shared__ double red[256];
     double acc = 0.0;
+    const int ng = G.ng;
     for (int k = threadIdx.x; k < ng; k += 256) {
-        const long long o = (long long)gj[k] * pitch + gi[k];
-        const double e = h[o] + bed[o];
+        const double e = spec_eta(G, h, bed, pitch, k);
         if (eta_out) eta_out[k] = e;
         if (tgt) {
             const double m = e - tgt[k];
@@ -508,23 +527,27 @@ __global__ __launch_bounds__(256) void k_gauge_loss(const double* __restrict__ h
 }
 
 // Seeds d L / d U^{n+1} at the gauges.  One thread per distinct gauge cell
-// adds that cell's gauges (cell_gauges[cell_start[u] .. cell_start[u+1]))
-// in gauge order, as a single thread walking all gauges would.
+// (cell_pos[u]) adds its taps (cell_gauges[q], weight cell_w[q] for q in
+// cell_start[u] .. cell_start[u+1]) in gauge order, as a single thread
+// walking all gauges would.  A cell gauge has one tap of weight 1; a
+// station the four of its bilinear weights ((1-wx)(1-wy), wx(1-wy),
+// (1-wx)wy, wx wy), since d eta / d h = d eta / d b = weight there.
 __global__ __launch_bounds__(256) void k_gauge_seed(const double* __restrict__ h, const double* __restrict__ bed,
-                                                   long long pitch, const int* __restrict__ gi,
-                                                   const int* __restrict__ gj, const double* __restrict__ tgt,
+                                                   long long pitch, GaugeSpec G, const double* __restrict__ tgt,
                                                    const int* __restrict__ cell_start,
-                                                   const int* __restrict__ cell_gauges, int ncells, double scale,
+                                                   const int* __restrict__ cell_gauges,
+                                                   const double* __restrict__ cell_w,
+                                                   const long long* __restrict__ cell_pos, int ncells, double scale,
                                                    double* __restrict__ lam_h, double* __restrict__ grad_b) {
     const int u = blockIdx.x * 256 + threadIdx.x;
     if (u >= ncells) return;
-    const int k0 = cell_gauges[cell_start[u]];
-    const long long o = (long long)gj[k0] * pitch + gi[k0];
-    const double e = h[o] + bed[o];
+    const long long o = cell_pos[u];
     double lh = lam_h[o], gb = grad_b[o];
     for (int q = cell_start[u]; q < cell_start[u + 1]; ++q) {
         const int k = cell_gauges[q];
-        const double sk = 2.0 * scale * (e - tgt[k]);
+        const double e = spec_eta(G, h, bed, pitch, k);
+        double sk = 2.0 * scale * (e - tgt[k]);
+        if (G.gw != nullptr) sk = sk * cell_w[q];
         lh += sk;
         gb += sk;
     }
@@ -624,8 +647,11 @@ struct rkg_adjoint {
     int nsteps, K, nseg, ng;
     int* gi;                    // device
     int* gj;
+    double* gw;                 // device: station weights wx then wy, or null (cell gauges)
     int* cell_start;            // device CSR of the distinct gauge cells (ncells + 1)
-    int* cell_gauges;           // device: gauge indices per cell, in gauge order (ng)
+    int* cell_gauges;           // device: gauge of each tap, per cell in gauge order
+    double* cell_w;             // device: weight of each tap (stations)
+    long long* cell_pos;        // device: offset of each distinct cell in a plane
     int ncells;
     double* tgt;                // device, nsteps x ng
     double* eta;                // device, nsteps x ng (gauge eta of the last forward run)
@@ -711,23 +737,12 @@ int body_blocks(const rkg_grid* g) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_i, const int* gauge_j,
-                       const double* targets, rkg_adjoint** out) {
-    if (!out) { set_error("null argument"); return RKG_ERR_RANGE; }
-    *out = nullptr;
-    if (!g || g->magic != 0x72616B47u) return RKG_ERR_BAD_HANDLE;
-    if (nsteps < 1 || K < 1 || ng < 0 || (ng > 0 && (!gauge_i || !gauge_j))) {
-        set_error("need nsteps >= 1, checkpoint_interval >= 1 and gauge indices");
-        return RKG_ERR_RANGE;
-    }
-    for (int k = 0; k < ng; ++k)
-        if (gauge_i[k] < 0 || gauge_i[k] >= g->cfg.nx || gauge_j[k] < 0 || gauge_j[k] >= g->cfg.ny) {
-            set_error("gauge %d at (%d, %d) lies outside the %dx%d grid", k, gauge_i[k], gauge_j[k], g->cfg.nx,
-                      g->cfg.ny);
-            return RKG_ERR_RANGE;
-        }
+// A session over gauges given as lower-left cells gauge_i / gauge_j and, for
+// stations, the weights wx / wy (null for cell gauges).
+int create_session(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_i, const int* gauge_j,
+                   const double* wx, const double* wy, const double* targets, rkg_adjoint** out) {
     if (g->scheme != RKG_TIME_RK2) {
         set_error("the adjoint is that of the SSP-RK2 step; set time scheme rk2");
         return RKG_ERR_UNSUPPORTED;
@@ -757,26 +772,49 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
         e = hipMalloc((void**)&a->gi, ng * sizeof(int));
         if (e == hipSuccess) e = hipMalloc((void**)&a->gj, ng * sizeof(int));
         if (e == hipSuccess) e = hipMemcpy(a->gi, gauge_i, ng * sizeof(int), hipMemcpyHostToDevice);
-        // distinct gauge cells in order of first appearance, each with its gauges in order
+        // taps: one per cell gauge (weight 1), four per station (its
+        // bilinear weights, gauges_write :519-537); distinct cells in order
+        // of first appearance, each with its taps in gauge order
+        const int nx = g->cfg.nx, ny = g->cfg.ny;
         std::vector<long long> keys;
-        std::vector<std::vector<int>> members;
-        for (int k = 0; k < ng; ++k) {
-            const long long key = (long long)gauge_j[k] * g->cfg.nx + gauge_i[k];
+        std::vector<std::vector<std::pair<int, double>>> members;
+        auto tap = [&](int k, int c, int j, double w) {
+            const long long key = (long long)j * g->pitch + c;
             size_t u = 0;
             while (u < keys.size() && keys[u] != key) ++u;
             if (u == keys.size()) { keys.push_back(key); members.emplace_back(); }
-            members[u].push_back(k);
+            members[u].push_back({k, w});
+        };
+        for (int k = 0; k < ng; ++k) {
+            if (!wx) { tap(k, gauge_i[k], gauge_j[k], 1.0); continue; }
+            const int c0 = gauge_i[k], j0 = gauge_j[k];
+            const int c1 = std::min(c0 + 1, nx - 1), j1 = std::min(j0 + 1, ny - 1);
+            tap(k, c0, j0, (1.0 - wx[k]) * (1.0 - wy[k]));
+            tap(k, c1, j0, wx[k] * (1.0 - wy[k]));
+            tap(k, c0, j1, (1.0 - wx[k]) * wy[k]);
+            tap(k, c1, j1, wx[k] * wy[k]);
         }
         std::vector<int> start(1, 0), list;
+        std::vector<double> wlist;
         for (const auto& m : members) {
-            list.insert(list.end(), m.begin(), m.end());
+            for (const auto& t : m) { list.push_back(t.first); wlist.push_back(t.second); }
             start.push_back((int)list.size());
         }
         a->ncells = (int)keys.size();
         if (e == hipSuccess) e = hipMalloc((void**)&a->cell_start, start.size() * sizeof(int));
         if (e == hipSuccess) e = hipMalloc((void**)&a->cell_gauges, list.size() * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&a->cell_w, wlist.size() * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void**)&a->cell_pos, keys.size() * sizeof(long long));
         if (e == hipSuccess) e = hipMemcpy(a->cell_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(a->cell_gauges, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(a->cell_w, wlist.data(), wlist.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(a->cell_pos, keys.data(), keys.size() * sizeof(long long), hipMemcpyHostToDevice);
+        if (wx) {
+            std::vector<double> w(wx, wx + ng);
+            w.insert(w.end(), wy, wy + ng);
+            if (e == hipSuccess) e = hipMalloc((void**)&a->gw, 2 * (size_t)ng * sizeof(double));
+            if (e == hipSuccess) e = hipMemcpy(a->gw, w.data(), 2 * (size_t)ng * sizeof(double), hipMemcpyHostToDevice);
+        }
         if (e == hipSuccess) e = hipMemcpy(a->gj, gauge_j, ng * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc((void**)&a->tgt, (size_t)nsteps * ng * sizeof(double));
         if (e == hipSuccess) e = hipMalloc((void**)&a->eta, (size_t)nsteps * ng * sizeof(double));
@@ -812,12 +850,54 @@ int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_
     return RKG_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int rkg_adjoint_create(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_i, const int* gauge_j,
+                       const double* targets, rkg_adjoint** out) {
+    if (!out) { set_error("null argument"); return RKG_ERR_RANGE; }
+    *out = nullptr;
+    if (!grid_valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (nsteps < 1 || K < 1 || ng < 0 || (ng > 0 && (!gauge_i || !gauge_j))) {
+        set_error("need nsteps >= 1, checkpoint_interval >= 1 and gauge indices");
+        return RKG_ERR_RANGE;
+    }
+    for (int k = 0; k < ng; ++k)
+        if (gauge_i[k] < 0 || gauge_i[k] >= g->cfg.nx || gauge_j[k] < 0 || gauge_j[k] >= g->cfg.ny) {
+            set_error("gauge %d at (%d, %d) lies outside the %dx%d grid", k, gauge_i[k], gauge_j[k], g->cfg.nx,
+                      g->cfg.ny);
+            return RKG_ERR_RANGE;
+        }
+    return create_session(g, nsteps, K, ng, gauge_i, gauge_j, nullptr, nullptr, targets, out);
+}
+
+int rkg_adjoint_create_stations(rkg_grid* g, int nsteps, int K, int n, const double* x, const double* y,
+                                const double* targets, rkg_adjoint** out) {
+    if (!out) { set_error("null argument"); return RKG_ERR_RANGE; }
+    *out = nullptr;
+    if (!grid_valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (nsteps < 1 || K < 1 || n < 0 || n > RKG_MAX_GAUGES || (n > 0 && (!x || !y))) {
+        set_error("need nsteps >= 1, checkpoint_interval >= 1 and station coordinates");
+        return RKG_ERR_RANGE;
+    }
+    std::vector<int> ci(n), cj(n);
+    std::vector<double> wx(n), wy(n);
+    for (int k = 0; k < n; ++k)
+        if (!station_weights(g->cfg, x[k], y[k], &ci[k], &cj[k], &wx[k], &wy[k])) {
+            set_error("station %d at (%g, %g) m lies outside the domain", k, x[k], y[k]);
+            return RKG_ERR_RANGE;
+        }
+    return create_session(g, nsteps, K, n, ci.data(), cj.data(), wx.data(), wy.data(), targets, out);
+}
+
 void rkg_adjoint_destroy(rkg_adjoint** pa) {
     if (!pa || !*pa) return;
     rkg_adjoint* a = *pa;
     if (a->magic != ADJ_MAGIC) { *pa = nullptr; return; }
     if (a->g && a->g->stream) (void)hipStreamSynchronize(a->g->stream);
-    for (void* p : {(void*)a->gi, (void*)a->gj, (void*)a->cell_start, (void*)a->cell_gauges, (void*)a->tgt, (void*)a->eta, (void*)a->ckpt, (void*)a->ckpt_ctrl,
+    for (void* p : {(void*)a->gi, (void*)a->gj, (void*)a->gw, (void*)a->cell_start, (void*)a->cell_gauges,
+                    (void*)a->cell_w, (void*)a->cell_pos, (void*)a->tgt, (void*)a->eta, (void*)a->ckpt, (void*)a->ckpt_ctrl,
                     (void*)a->ws, (void*)a->lam, (void*)a->hist, (void*)a->loss_dev, (void*)a->part,
                     (void*)a->part_idx, (void*)a->scal, (void*)a->edge_lg})
         if (p) (void)hipFree(p);
@@ -849,6 +929,10 @@ int rkg_adjoint_gauge_eta(rkg_adjoint* a, double* eta) {
 }  // extern "C"
 
 namespace {
+
+GaugeSpec gauge_spec(const rkg_adjoint* a) {
+    return GaugeSpec{a->gi, a->gj, a->gw, a->ng, a->g->cfg.nx, a->g->cfg.ny};
+}
 
 // --- the forward run and the reverse sweep, as per-step pieces that queue
 // work on the session's stream without waiting, so that several sessions on
@@ -894,7 +978,7 @@ int fwd_step(rkg_adjoint* a, int n) {
     if (rc) return rc;
     if (a->ng > 0)
         hipLaunchKernelGGL(k_gauge_loss, dim3(1), dim3(256), 0, g->stream, g->buf, g->buf + 6 * Pl,
-                           (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng, a->ng,
+                           (long long)g->pitch, gauge_spec(a), a->tgt + (size_t)n * a->ng,
                            a->eta + (size_t)n * a->ng, a->loss_dev);
     return RKG_OK;
 }
@@ -989,8 +1073,9 @@ void bwd_reverse_step(rkg_adjoint* a, int s, int k, const BwdPlan& pl) {
     double* pval = a->part + 2 * nb;
     if (a->ng > 0)
         hipLaunchKernelGGL(k_gauge_seed, dim3((a->ncells + 255) / 256), dim3(256), 0, g->stream, wsA(a, k + 1),
-                           g->buf + 6 * Pl, (long long)g->pitch, a->gi, a->gj, a->tgt + (size_t)n * a->ng,
-                           a->cell_start, a->cell_gauges, a->ncells, pl.scale, lamA(a), gradB(a));
+                           g->buf + 6 * Pl, (long long)g->pitch, gauge_spec(a), a->tgt + (size_t)n * a->ng,
+                           a->cell_start, a->cell_gauges, a->cell_w, a->cell_pos, a->ncells, pl.scale, lamA(a),
+                           gradB(a));
     AdjStage st;
     // stage 2: input U* (B_k), U^n = A_k
     st.d = stage_desc(a, wsB(a, k), wsB(a, k), wsA(a, k + 1));

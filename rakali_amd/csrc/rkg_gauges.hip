@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/rakali_amd.h"
+#include "gauge_eval.hpp"
 #include "rkg_internal.hpp"
 
 namespace rk {
@@ -44,25 +45,16 @@ struct GaugeState {
 
 namespace {
 
-__device__ __forceinline__ double bilin(const double* f, long long pitch, int c0, int c1, int j0, int j1, double wx,
-                                        double wy) {
-    // gauges_write :519-537, in the reference's operation order
-    return (1.0 - wx) * (1.0 - wy) * f[(long long)j0 * pitch + c0] + wx * (1.0 - wy) * f[(long long)j0 * pitch + c1]
-           + (1.0 - wx) * wy * f[(long long)j1 * pitch + c0] + wx * wy * f[(long long)j1 * pitch + c1];
-}
-
-// Station g of the current state A.  The upper neighbour of a station in
-// the edge half cell is the ghost in the reference, with weight 0: the
-// interior cell stands in for it (the product is a zero either way).
+// Station g of the current state A (gauges_write, rki_gauges.F90:516-550).
 __device__ __forceinline__ void gauge_value(const GridDesc& d, const GaugeTable& t, int g, double& eta, double& u,
                                             double& v) {
     const int c0 = t.c0[g], j0 = t.j0[g];
     const int c1 = min(c0 + 1, d.nx - 1), j1 = min(j0 + 1, d.ny - 1);
     const double wx = t.wx[g], wy = t.wy[g];
-    const double hi = bilin(d.ah, d.pitch, c0, c1, j0, j1, wx, wy);
-    const double hui = bilin(d.ahu, d.pitch, c0, c1, j0, j1, wx, wy);
-    const double hvi = bilin(d.ahv, d.pitch, c0, c1, j0, j1, wx, wy);
-    const double bi = bilin(d.bed, d.pitch, c0, c1, j0, j1, wx, wy);
+    const double hi = station_bilinear(d.ah, d.pitch, c0, c1, j0, j1, wx, wy);
+    const double hui = station_bilinear(d.ahu, d.pitch, c0, c1, j0, j1, wx, wy);
+    const double hvi = station_bilinear(d.ahv, d.pitch, c0, c1, j0, j1, wx, wy);
+    const double bi = station_bilinear(d.bed, d.pitch, c0, c1, j0, j1, wx, wy);
     eta = hi + bi;
     const double hs = fmax(hi, 1.0e-6);   // MIN_DEPTH, :508
     u = hui / hs;
@@ -142,6 +134,28 @@ void free_tables(GaugeState* s) {
 
 }  // namespace
 
+bool station_weights(const rkg_config& c, double x, double y, int* c0, int* j0, double* wx, double* wy) {
+    const int ig = cell_of(x, c.dx, c.nx), jg = cell_of(y, c.dy, c.ny);
+    if (!(ig >= 1 && ig <= c.nx && jg >= 1 && jg <= c.ny)) return false;
+    const double xc = ((double)ig - 0.5) * c.dx, yc = ((double)jg - 0.5) * c.dy;
+    double u, v;
+    if (x < xc) {
+        if (ig > 1) { *c0 = ig - 2; u = (x - (xc - c.dx)) / c.dx; } else { *c0 = ig - 1; u = 0.0; }
+    } else {
+        *c0 = ig - 1;
+        u = ig < c.nx ? (x - xc) / c.dx : 0.0;
+    }
+    if (y < yc) {
+        if (jg > 1) { *j0 = jg - 2; v = (y - (yc - c.dy)) / c.dy; } else { *j0 = jg - 1; v = 0.0; }
+    } else {
+        *j0 = jg - 1;
+        v = jg < c.ny ? (y - yc) / c.dy : 0.0;
+    }
+    *wx = std::max(0.0, std::min(1.0, u));
+    *wy = std::max(0.0, std::min(1.0, v));
+    return true;
+}
+
 long long gauges_room(const rkg_grid* g) {
     const GaugeState* s = g->gauges;
     if (!s || !s->recording || s->n_local == 0) return 1ll << 40;
@@ -207,36 +221,28 @@ int rkg_gauges_set(rkg_grid* g, int n, const double* x, const double* y, int* n_
     const rkg_config& c = g->cfg;
     std::vector<int> cell;
     std::vector<double> w;
-    for (int k = 0; k < n; ++k) {   // compute_local_gauges, rki_gauges.F90:208-243
-        const int ig = cell_of(x[k], c.dx, c.nx), jg = cell_of(y[k], c.dy, c.ny);
-        if (ig >= 1 && ig <= c.nx && jg >= 1 && jg <= c.ny) s->local_idx.push_back(k);
+    for (int k = 0; k < n; ++k) {
+        int c0, j0;
+        double wx, wy;
+        if (station_weights(c, x[k], y[k], &c0, &j0, &wx, &wy)) {
+            s->local_idx.push_back(k);
+            cell.push_back(c0);
+            cell.push_back(j0);
+            w.push_back(wx);
+            w.push_back(wy);
+        }
     }
     const int nl = (int)s->local_idx.size();
     s->n_local = nl;
-    cell.resize(2 * (size_t)nl);
-    w.resize(2 * (size_t)nl);
-    for (int l = 0; l < nl; ++l) {   // compute_interp_weights, :245-336 (here 0-based interior cells)
-        const double gx = x[s->local_idx[l]], gy = y[s->local_idx[l]];
-        const int ig = cell_of(gx, c.dx, c.nx), jg = cell_of(gy, c.dy, c.ny);
-        const double xc = ((double)ig - 0.5) * c.dx, yc = ((double)jg - 0.5) * c.dy;
-        int c0, j0;
-        double wx, wy;
-        if (gx < xc) {
-            if (ig > 1) { c0 = ig - 2; wx = (gx - (xc - c.dx)) / c.dx; } else { c0 = ig - 1; wx = 0.0; }
-        } else {
-            c0 = ig - 1;
-            wx = ig < c.nx ? (gx - xc) / c.dx : 0.0;
+    {   // (c0, j0) pairs and (wx, wy) pairs -> the table's planar layout
+        std::vector<int> cp(2 * (size_t)nl);
+        std::vector<double> wp(2 * (size_t)nl);
+        for (int l = 0; l < nl; ++l) {
+            cp[l] = cell[2 * l]; cp[nl + l] = cell[2 * l + 1];
+            wp[l] = w[2 * l]; wp[nl + l] = w[2 * l + 1];
         }
-        if (gy < yc) {
-            if (jg > 1) { j0 = jg - 2; wy = (gy - (yc - c.dy)) / c.dy; } else { j0 = jg - 1; wy = 0.0; }
-        } else {
-            j0 = jg - 1;
-            wy = jg < c.ny ? (gy - yc) / c.dy : 0.0;
-        }
-        cell[l] = c0;
-        cell[nl + l] = j0;
-        w[l] = std::max(0.0, std::min(1.0, wx));
-        w[nl + l] = std::max(0.0, std::min(1.0, wy));
+        cell.swap(cp);
+        w.swap(wp);
     }
     if (nl > 0) {
         HIP_TRY(hipMalloc((void**)&s->d_cell, 2 * (size_t)nl * sizeof(int)));

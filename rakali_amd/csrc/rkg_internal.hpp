@@ -65,6 +65,12 @@ inline double* plane_ptr(rkg_grid* g, int k) { return g->buf + (size_t)k * g->pl
 // Gauge recording around the step launches (rkg_gauges.hip): the steps one
 // launch may take before the device series must be drained, the drain, the
 // record after a step (slot p holds the state after it), and the release.
+// Station (x, y) in metres: false when it lies outside the domain
+// (compute_local_gauges, rki_gauges.F90:208-243); else its lower-left
+// interpolation cell (c0, j0, 0-based interior) and weights
+// (compute_interp_weights, :245-336).
+bool station_weights(const rkg_config& c, double x, double y, int* c0, int* j0, double* wx, double* wy);
+
 long long gauges_room(const rkg_grid* g);
 int gauges_drain(rkg_grid* g);
 int gauges_after_step(rkg_grid* g, int p);

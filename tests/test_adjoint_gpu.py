@@ -246,3 +246,84 @@ def test_batched_sessions_equal_single_sessions_bitwise():
         s.close()
     for g in grids:
         g.close()
+
+
+def test_stations_at_cell_centres_equal_cell_gauges():
+    """A station at a cell centre has weights (0, 0): the same loss bit for
+    bit and the same gradients as the cell gauge."""
+    from rakali_amd.adjoint import GaugeMisfit
+    nx, ny, steps = 60, 40, 9
+    rng, b, n, h, hu, hv = generic_state(nx, ny, 11)
+    cells = [(3, 4), (20, 30), (59, 0), (0, 39), (33, 17)]
+    tgt = rng.normal(0.0, 0.05, (steps, len(cells)))
+    out = []
+    for kind in ("cells", "stations"):
+        g = gpu_grid(nx, ny, dict(CFG, dt=0.0), True, ("tidal", "wall", "open", "wall"), 1e-3)
+        if kind == "cells":
+            sess = GaugeMisfit(g, steps, 4, cells, tgt)
+        else:
+            xs = [(i + 0.5) * CFG["dx"] for i, _ in cells]
+            ys = [(j + 0.5) * CFG["dy"] for _, j in cells]
+            sess = GaugeMisfit(g, steps, 4, targets=tgt, stations=(xs, ys))
+        load(g, b, n, h, hu, hv)
+        loss = sess.forward()
+        sess.backward()
+        out.append((loss, {k: sess.gradient(k) for k in ("b", "n", "h", "hu", "hv")}))
+        sess.close()
+        g.close()
+    assert out[0][0] == out[1][0]
+    for k in out[0][1]:
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+
+
+@pytest.mark.parametrize("dt", [0.4, 0.0])
+def test_station_loss_directional_derivatives_and_recorder(dt):
+    """Bilinear stations (random points, points in the edge half cells):
+    the session's eta equals the gauge recorder's series bit for bit, and
+    the gradient matches central differences of the GPU forward.  Each
+    direction takes the differences at two steps (1e-7 and 1e-8, or 1e-8
+    and 1e-9 with the CFL dt): a kink of the scheme (a minmod or wave-speed
+    tie) within one step of the state throws that quotient off by 1e-5 or
+    more, while the other still agrees to about 1e-8, so the closer of the
+    two is held to 1e-6."""
+    from rakali_amd.adjoint import GaugeMisfit
+    nx, ny, steps = 70, 45, 12
+    rng, b, n, h, hu, hv = generic_state(nx, ny, 8)
+    g = gpu_grid(nx, ny, dict(CFG, dt=dt), True, ("tidal", "wall", "open", "wall"), 1e-3)
+    W, H = nx * CFG["dx"], ny * CFG["dy"]
+    xs = list(rng.uniform(0.0, W, 14)) + [0.2 * CFG["dx"], W - 0.3 * CFG["dx"]]
+    ys = list(rng.uniform(0.0, H, 14)) + [H - 0.1 * CFG["dy"], 0.4 * CFG["dy"]]
+    sess = GaugeMisfit(g, steps, 5, targets=rng.normal(0.0, 0.05, (steps, 16)), stations=(xs, ys))
+    load(g, b, n, h, hu, hv)
+    g.set_gauges(xs, ys)
+    g.record_gauges(0.0, 64)
+    loss = sess.forward()
+    ser = g.gauge_series()
+    assert np.array_equal(ser["eta"], sess.gauge_eta())
+    g.record_gauges(0.0, 0)
+    sess.backward()
+    gr = {k: sess.gradient(k) for k in ("b", "n", "h", "hu", "hv")}
+    worst = 0.0
+    for _ in range(4):
+        d = {k: rng.normal(size=b.shape) for k in gr}
+        d["n"] *= 0.005
+        for k in ("h", "hu", "hv"):
+            d[k] *= h > 0.05
+        ad = sum(float(np.sum(gr[k] * d[k])) for k in gr)
+        errs = []
+        for eps in ((1e-7, 1e-8) if dt > 0.0 else (1e-8, 1e-9)):
+            vals = []
+            for s in (1.0, -1.0):
+                load(g, b + s * eps * d["b"], n + s * eps * d["n"], h + s * eps * d["h"],
+                     hu + s * eps * d["hu"], hv + s * eps * d["hv"])
+                vals.append(sess.forward())
+            fd = (vals[0] - vals[1]) / (2 * eps)
+            print(f"  eps {eps:.0e}: fd {fd:.12e}  adjoint {ad:.12e}", flush=True)
+            errs.append(abs(fd - ad) / abs(ad))
+        worst = max(worst, min(errs))
+    print(f"[station adjoint] dt={dt} loss={loss:.6e} worst rel {worst:.2e}", flush=True)
+    assert worst < 1e-6
+    with pytest.raises(Exception, match="outside"):
+        GaugeMisfit(g, steps, 5, stations=([W + 1.0], [1.0]))
+    sess.close()
+    g.close()
