@@ -12,8 +12,10 @@ fallback, and a missing library raises ``FileNotFoundError``.
 
 Differences, all from features this build does not run: multilayer and
 non-hydrostatic simulations, nested and Chapman edges and the
-semi-implicit scheme raise ``RuntimeError`` (the library returns -5); ``save_restart`` writes the library's binary restart
-format instead of NetCDF; ``to_xarray``/``save_checkpoint`` need xarray.
+semi-implicit scheme raise ``RuntimeError`` (the library returns -5);
+``save_restart`` writes the library's binary restart format instead of
+NetCDF; ``save_checkpoint`` writes classic NetCDF with scipy;
+``to_xarray`` needs xarray.
 """
 from __future__ import annotations
 
@@ -604,11 +606,49 @@ class Rakali:
         return sim
 
     def save_checkpoint(self, filename):
-        try:
-            import xarray  # noqa: F401
-        except ImportError as e:
-            raise ImportError("save_checkpoint() requires the xarray package.") from e
-        raise NotImplementedError("save_checkpoint writes NetCDF, which this build does not")
+        """Write the single-step mid-RK checkpoint (_core.py:1433-1506): the
+        ghost-included h, hu, hv, b and the RK2 saves h0, hu0, hv0 on
+        dimensions (x_total, y_total), with the time, step count, dt, grid
+        and physics scalars as global attributes.  The file is NetCDF, the
+        classic 64-bit-offset format written with scipy (xarray, which the
+        reference writes it with, and its zlib compression are not needed
+        to read it back)."""
+        from scipy.io import netcdf_file
+
+        self._check_ready()
+        self.sync_host()
+        _require_writable_dir(filename)
+        nxt, nyt = self._nx_phys + 2 * self._nghost, self._ny_phys + 2 * self._nghost
+        fields = {
+            "h": (self.h, "m", "water depth (ghost-included)"),
+            "hu": (self.hu, "m2 s-1", "x-momentum (ghost-included)"),
+            "hv": (self.hv, "m2 s-1", "y-momentum (ghost-included)"),
+            "b": (self.b, "m", "bottom elevation"),
+            "h0": (self.h0, "m", "RK2 save of h at start of stage 1"),
+            "hu0": (self.hu0, "m2 s-1", "RK2 save of hu at start of stage 1"),
+            "hv0": (self.hv0, "m2 s-1", "RK2 save of hv at start of stage 1"),
+        }
+        attrs = {
+            "title": "Rakali per-step checkpoint (2D barotropic)",
+            "source": "rakali_amd.Rakali.save_checkpoint",
+            "time": float(self.time), "step_count": int(self.step_count), "dt": float(self.dt),
+            "nx_total": int(nxt), "ny_total": int(nyt), "nx_phys": int(self._nx_phys),
+            "ny_phys": int(self._ny_phys), "nghost": int(self._nghost), "dx": float(self._dx),
+            "dy": float(self._dy), "manning_n": float(self.manning_n), "coriolis_f": float(self.coriolis_f),
+            "clamped_elevation": float(self.clamped_elevation),
+        }
+        with netcdf_file(str(filename), "w", version=2) as nc:
+            for k, v in attrs.items():
+                setattr(nc, k, np.int32(v) if isinstance(v, int) else v)
+            nc.createDimension("x_total", nxt)
+            nc.createDimension("y_total", nyt)
+            for name, (arr, units, long_name) in fields.items():
+                var = nc.createVariable(name, "d", ("x_total", "y_total"))
+                var[:, :] = np.asarray(arr)
+                var.units = units
+                var.long_name = long_name
+                if name == "b":
+                    var.positive = "up"
 
     def to_xarray(self, time_dim=True):
         try:

@@ -435,3 +435,27 @@ def test_fortran_driver_gauges(tmp_path):
         eta, u, v = G.sample(*(np.asfortranarray(a) for a in (sim.h, sim.hu, sim.hv, sim.b)))
     last = rows[-1, 1:].reshape(3, 3)
     assert np.array_equal(last[:, 0], eta) and np.array_equal(last[:, 1], u) and np.array_equal(last[:, 2], v)
+
+
+def test_save_checkpoint_netcdf(tmp_path):
+    """save_checkpoint (_core.py:1433-1506): the ghost-included state and RK
+    saves and the scalars, read back from the NetCDF file bit for bit."""
+    from scipy.io import netcdf_file
+    from rakali_amd import Rakali
+    cfg = dict(nx=30, ny=20, dx=5.0, dy=4.0, t_end=50.0, bc_west="tidal", tidal_amplitude=0.3,
+               tidal_period=60.0, init_condition="gaussian_hump", hump_sigma=20.0, hump_amplitude=0.3,
+               manning_n=0.02, coriolis_f=1e-4, use_hllc=True)
+    with Rakali(**cfg) as sim:
+        sim.step(7)
+        path = tmp_path / "ck.nc"
+        sim.save_checkpoint(path)
+        ng = (sim.h.shape[0] - 30) // 2
+        with netcdf_file(str(path), "r", mmap=False) as nc:
+            for k in ("h", "hu", "hv", "b", "h0", "hu0", "hv0"):
+                got = nc.variables[k][:].copy()
+                assert got.shape == (30 + 2 * ng, 20 + 2 * ng)
+                assert np.array_equal(got, getattr(sim, k)), k
+            assert nc.time == sim.time and nc.dt == sim.dt and nc.step_count == 7
+            assert (nc.nx_total, nc.ny_total, nc.nghost) == (30 + 2 * ng, 20 + 2 * ng, ng)
+            assert nc.manning_n == 0.02 and nc.coriolis_f == 1e-4 and nc.dx == 5.0 and nc.dy == 4.0
+            assert nc.variables["b"].positive == b"up"
