@@ -9,8 +9,8 @@
  * finalised; -2 shape mismatch, already finalised or out of range; -3
  * multilayer inactive or restart mismatch; -100 invalid handle.  Two codes
  * are additions: -4 a HIP runtime error, -5 a configuration this build
- * does not implement (multilayer, non-hydrostatic, nested / Chapman
- * edges, the semi-implicit scheme, NetCDF bathymetry files).  Where the reference ends the process with
+ * does not implement (multilayer, non-hydrostatic, nested edges, the
+ * semi-implicit scheme, NetCDF bathymetry files).  Where the reference ends the process with
  * ERROR STOP (an invalid namelist), these functions return -2 and
  * rki_last_error() describes the problem.
  *

@@ -51,6 +51,12 @@ extern "C" {
 #define RKG_BC_DISCHARGE 6
 #define RKG_BC_CLAMPED 7
 #define RKG_BC_SPONGE 8
+/* Chapman / Orlanski radiation (bc_chapman_edge, rki_boundary.F90:471-552):
+ * the ghost surface moves with the phase speed estimated from the interior
+ * eta tendency since the previous boundary call and the along-normal
+ * gradient; the normal momentum is the Flather radiation about it.  The
+ * adjoint and the ensembles do not support it. */
+#define RKG_BC_CHAPMAN 9
 
 /* Field selectors.  H0/HU0/HV0 are U^n of the last step (the reference's
  * RK save h0/hu0/hv0, rki_kernel_fused.F90:224-226), kept only while

@@ -56,6 +56,7 @@ class Params(ctypes.Structure):
         ("forcing_active", ctypes.c_int),
         ("sponge_width", ctypes.c_int),
         ("sponge_strength", ctypes.c_double), ("sponge_h_ref", ctypes.c_double),
+        ("chap_eta_old", ctypes.c_void_p * 4),
     ]
 
 
@@ -100,6 +101,8 @@ def lib():
         L.orc_tidal_eta.argtypes = [P, ctypes.c_double]
         L.orc_tidal_eta.restype = ctypes.c_double
         L.orc_apply_bc.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
+        L.orc_apply_bc_dt.argtypes = [P, ctypes.c_double, ctypes.c_double, dp, dp, dp, dp]
+        L.orc_chapman_seed.argtypes = [P, dp, dp]
         L.orc_fill_bathymetry_ghosts.argtypes = [P, dp]
         L.orc_init_still_water.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
         L.orc_total_mass.argtypes = [P, dp]
@@ -249,6 +252,14 @@ class Sim:
 
     def finalize(self):
         fill_bathymetry_ghosts(self.p, self.b)
+        # Chapman edges: eta_old per edge (W/E ny_total, S/N nx_total),
+        # seeded by bc_init_chapman (rki_api.F90:250, after the ghosts)
+        if any(self.p.bc[s] == BC_NAMES["chapman"] for s in range(4)):
+            nxt, nyt = self.p.nx + 2 * self.p.ng, self.p.ny + 2 * self.p.ng
+            self.chap_eta_old = [np.zeros(nyt), np.zeros(nyt), np.zeros(nxt), np.zeros(nxt)]
+            for s in range(4):
+                self.p.chap_eta_old[s] = self.chap_eta_old[s].ctypes.data
+            lib().orc_chapman_seed(ctypes.byref(self.p), _ptr(self.h), _ptr(self.b))
 
     def step(self, n: int) -> int:
         return lib().orc_run(ctypes.byref(self.p), _ptr(self.h), _ptr(self.hu),

@@ -825,10 +825,82 @@ static void bc_simple_edge(const orc_params *p, double *h, double *hu, double *h
     }
 }
 
+/* bc_chapman_edge, rki_boundary.F90:471-552: Chapman / Orlanski radiation.
+ * The ghost surface advects at the phase speed c = -(d eta/dt)/(d eta/dn)
+ * from the interior tendency against eta_old and the along-normal
+ * gradient, clamped to [0, sqrt(g hmax)]; the normal momentum is the
+ * Flather radiation about that surface.  The loops run t outer, k inner
+ * (the serial order of the reference's CPU build), so eta_old(t), set to
+ * the interior eta on the k = 1 layer, is already current for k = 2. */
+static void bc_chapman_edge(const orc_params *p, double *h, double *hu, double *hv,
+                            const double *b, const edge_t *e, double *eta_old, double dt,
+                            double dd)
+{
+    const int nxt = NT(p), nyt = MT(p), ng = p->ng;
+    const double g = p->g, sgn = e->nrm_sign;
+    for (int t = 1; t <= e->n_tan; ++t) {
+        for (int k = 1; k <= ng; ++k) {
+            const int ig = e->ig0 + (k - 1) * e->din + (t - 1) * e->dit;
+            const int jg = e->jg0 + (k - 1) * e->djn + (t - 1) * e->djt;
+            const int ii = e->ig0 + (t - 1) * e->dit - e->din;
+            const int ji = e->jg0 + (t - 1) * e->djt - e->djn;
+            const int i2 = ii - e->din, j2 = ji - e->djn;
+            const double h_int = h[IX(ii, ji)];
+            const double eta_int = h_int + b[IX(ii, ji)];
+            const double eta_old_val = eta_old[t - 1];
+            const double hmax = dmax(h_int, 1.0e-6);
+            const double speed = sqrt(g * hmax);
+            double eta_ghost;
+            if (dt > 0.0) {
+                const double deta_dt = (eta_int - eta_old_val) / dt;
+                double deta_dx = 0.0;
+                if (i2 >= 1 && i2 <= nxt && j2 >= 1 && j2 <= nyt)
+                    deta_dx = (h[IX(i2, j2)] + b[IX(i2, j2)] - eta_int) / dd;
+                double c_phase = fabs(deta_dx) > 1.0e-10 ? -deta_dt / deta_dx : 0.0;
+                c_phase = dmax(0.0, fmin(c_phase, speed));
+                eta_ghost = eta_old_val - c_phase * dt / dd * (eta_old_val - eta_int);
+            } else {
+                eta_ghost = eta_int;
+            }
+            if (k == 1) eta_old[t - 1] = eta_int;
+            const double h_ghost = fmax0(eta_ghost - b[IX(ig, jg)]);
+            h[IX(ig, jg)] = h_ghost;
+            if (e->normal_is_u) {
+                hu[IX(ig, jg)] = h_ghost * (hu[IX(ii, ji)] / hmax + sgn * speed * (eta_int - eta_ghost) / hmax);
+                hv[IX(ig, jg)] = hv[IX(ii, ji)];
+            } else {
+                hv[IX(ig, jg)] = h_ghost * (hv[IX(ii, ji)] / hmax + sgn * speed * (eta_int - eta_ghost) / hmax);
+                hu[IX(ig, jg)] = hu[IX(ii, ji)];
+            }
+        }
+    }
+}
+
+void orc_chapman_seed(const orc_params *p, const double *h, const double *b)
+{
+    const int nxt = NT(p), nyt = MT(p), ng = p->ng;
+    const int any_we = p->bc[0] == ORC_BC_CHAPMAN || p->bc[1] == ORC_BC_CHAPMAN;
+    const int any_sn = p->bc[2] == ORC_BC_CHAPMAN || p->bc[3] == ORC_BC_CHAPMAN;
+    for (int j = 1; j <= nyt && any_we; ++j) {
+        if (p->chap_eta_old[0]) p->chap_eta_old[0][j - 1] = h[IX(ng + 1, j)] + b[IX(ng + 1, j)];
+        if (p->chap_eta_old[1]) p->chap_eta_old[1][j - 1] = h[IX(nxt - ng, j)] + b[IX(nxt - ng, j)];
+    }
+    for (int i = 1; i <= nxt && any_sn; ++i) {
+        if (p->chap_eta_old[2]) p->chap_eta_old[2][i - 1] = h[IX(i, ng + 1)] + b[IX(i, ng + 1)];
+        if (p->chap_eta_old[3]) p->chap_eta_old[3][i - 1] = h[IX(i, nyt - ng)] + b[IX(i, nyt - ng)];
+    }
+}
+
 /* apply_boundary_conditions, rki_boundary.F90:40-218: edges W, E, S, N in
  * that order; W/E span the full ny_total and S/N the full nx_total. */
 void orc_apply_bc(const orc_params *p, double time, double *h, double *hu,
                   double *hv, double *b)
+{
+    orc_apply_bc_dt(p, time, 0.0, h, hu, hv, b);
+}
+
+void orc_apply_bc_dt(const orc_params *p, double time, double bc_dt, double *h,
+                     double *hu, double *hv, double *b)
 {
     const int nxt = NT(p), nyt = MT(p), ng = p->ng;
     const int i_first = ng + 1, i_last = nxt - ng, j_first = ng + 1, j_last = nyt - ng;
@@ -848,7 +920,11 @@ void orc_apply_bc(const orc_params *p, double time, double *h, double *hu,
         case ORC_BC_INFLOW:
         case ORC_BC_DISCHARGE:
         case ORC_BC_CLAMPED: bc_simple_edge(p, h, hu, hv, b, &e[s], p->bc[s]); break;
-        default: break; /* nested / chapman: not restated */
+        case ORC_BC_CHAPMAN:
+            if (p->chap_eta_old[s])
+                bc_chapman_edge(p, h, hu, hv, b, &e[s], p->chap_eta_old[s], bc_dt, s < 2 ? p->dx : p->dy);
+            break;
+        default: break; /* nested: not restated */
         }
     }
 }
@@ -963,8 +1039,9 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
             dt_max = target - clk->t;
         }
         double time = clk->t;
-        /* solver_step_impl :718-750 */
-        orc_apply_bc(p, time, h, hu, hv, b);
+        /* solver_step_impl :718-750; bc%dt still holds the previous step's
+         * dt (set at :748), 0 before the first step */
+        orc_apply_bc_dt(p, time, clk->dt, h, hu, hv, b);
         if (p->dt_fixed > 0.0) {
             clk->dt = dmin(p->dt_fixed, dt_max);
         } else if (clk->step_count % p->cfl_interval == 0 || clk->dt <= 0.0) {
@@ -981,7 +1058,7 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
             memcpy(hv0, hv, n * sizeof(double));
             for (int st = 0; st < n_stages; ++st) {
                 const double dt_stage = beta[st] * dt;
-                orc_apply_bc(p, time + cabs[st] * dt, h, hu, hv, b);
+                orc_apply_bc_dt(p, time + cabs[st] * dt, dt, h, hu, hv, b);
                 orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);
                 orc_fused_stage_generic(p, h, hu, hv, fh, fhu, fhv, dt_stage);
                 if (alpha[st] != 0.0) orc_ssp_combine(p, h, hu, hv, h0, hu0, hv0, alpha[st]);
@@ -991,8 +1068,8 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
             ++taken;
             continue;
         }
-        /* stage 1 :763-787 */
-        orc_apply_bc(p, time, h, hu, hv, b);
+        /* stage 1 :763-787, bc%dt = state%dt (:748) */
+        orc_apply_bc_dt(p, time, dt, h, hu, hv, b);
         orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);
         clk->mass_out = clk->mass_out - 0.5 * dt * interior_sum(p, fh) * area;
         orc_fused_stage1(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt);
@@ -1001,7 +1078,7 @@ long long orc_run(const orc_params *p, double *h, double *hu, double *hv,
         sponge_with_budget(p, clk, h, hu, hv, dt, area);   /* :787 */
         /* stage 2 :789-814, bc%time = time + dt */
         double time2 = time + dt;
-        orc_apply_bc(p, time2, h, hu, hv, b);
+        orc_apply_bc_dt(p, time2, dt, h, hu, hv, b);
         orc_compute_flux(p, h, hu, hv, b, fh, fhu, fhv, p->use_hllc);
         clk->mass_out = clk->mass_out - 0.5 * dt * interior_sum(p, fh) * area;
         orc_fused_stage2(p, h, hu, hv, h0, hu0, hv0, fh, fhu, fhv, dt);

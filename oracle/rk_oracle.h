@@ -49,6 +49,10 @@ typedef struct {
      * rki_boundary_types.F90:73-77); off when width <= 0 or strength <= 0. */
     int sponge_width;
     double sponge_strength, sponge_h_ref;
+    /* Chapman radiation edges: each edge's eta_old (bc_params_t
+     * eta_old_west/east/south/north, rki_boundary_types.F90; W/E of length
+     * ny_total, S/N nx_total), updated by every boundary call, or NULL. */
+    double *chap_eta_old[4];
 } orc_params;
 
 /* Explicit SSP schemes, tags as rki_timestepper.F90:27-33.  ORC_TIME_RK2
@@ -106,8 +110,16 @@ void orc_gauges_sample(const orc_params *p, const double *h, const double *hu,
 double orc_compute_timestep(const orc_params *p, const double *h,
                             const double *hu, const double *hv);
 double orc_tidal_eta(const orc_params *p, double time);
+/* apply_boundary_conditions at bc%time = time with bc%dt = 0 (its value
+ * before the first step); orc_apply_bc_dt with the step's dt (bc%dt =
+ * state%dt, rki_solver.F90:748), which the Chapman edges read. */
 void orc_apply_bc(const orc_params *p, double time, double *h, double *hu,
                   double *hv, double *b);
+void orc_apply_bc_dt(const orc_params *p, double time, double bc_dt, double *h,
+                     double *hu, double *hv, double *b);
+/* bc_init_chapman (rki_boundary_types.F90:155-202): seeds eta_old of the
+ * Chapman edges from h + b of the first interior column / row. */
+void orc_chapman_seed(const orc_params *p, const double *h, const double *b);
 void orc_fill_bathymetry_ghosts(const orc_params *p, double *b);
 void orc_init_still_water(const orc_params *p, double eta0, const double *b,
                           double *h, double *hu, double *hv);

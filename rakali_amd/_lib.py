@@ -24,7 +24,7 @@ ERRORS = {-1: "not ready", -2: "argument out of range", -4: "HIP runtime error",
           -5: "unsupported configuration", -100: "invalid handle"}
 
 BC = {"wall": 1, "open": 2, "tidal": 3, "inflow": 5, "discharge": 6,
-      "clamped": 7, "sponge": 8}
+      "clamped": 7, "sponge": 8, "chapman": 9}
 FIELDS = {"h": 0, "hu": 1, "hv": 2, "b": 3, "h0": 4, "hu0": 5, "hv0": 6,
           "h_star": 7, "hu_star": 8, "hv_star": 9, "n": 10,
           "wind_x": 11, "wind_y": 12, "atm_pressure": 13, "precipitation": 14}

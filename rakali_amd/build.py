@@ -23,7 +23,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "librakali_amd.so"
 SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip", CSRC / "rkg_adjoint.hip", CSRC / "rkg_gauges.hip",
-           *(CSRC / f"stage_feat{k}.hip" for k in range(4))]
+           *(CSRC / f"stage_feat{k}.hip" for k in range(8))]
 HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp", CSRC / "adjoint.hpp",
            CSRC / "stage_kernels.hpp", CSRC / "stage_launch.hpp", CSRC / "stage_feat.inc",
            CSRC / "rkg_internal.hpp", ROOT / "include" / "rakali_amd.h"]

@@ -755,6 +755,11 @@ int create_session(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_i, c
         set_error("the adjoint does not cover the sponge relaxation; set its width to 0");
         return RKG_ERR_UNSUPPORTED;
     }
+    for (int e = 0; e < 4; ++e)
+        if (g->cfg.bc[e] == RKG_BC_CHAPMAN) {
+            set_error("the adjoint does not cover Chapman edges");
+            return RKG_ERR_UNSUPPORTED;
+        }
     int rc = rkg_use_manning_field(g, 1);
     if (rc) return rc;
     rkg_adjoint* a = new rkg_adjoint();
@@ -952,6 +957,11 @@ int fwd_begin(rkg_adjoint* a) {
         set_error("the adjoint does not cover the sponge relaxation; set its width to 0");
         return RKG_ERR_UNSUPPORTED;
     }
+    for (int e = 0; e < 4; ++e)
+        if (g->cfg.bc[e] == RKG_BC_CHAPMAN) {
+            set_error("the adjoint does not cover Chapman edges");
+            return RKG_ERR_UNSUPPORTED;
+        }
     HIP_TRY(hipMemsetAsync(a->loss_dev, 0, sizeof(double), g->stream));
     HIP_TRY(hipEventRecord(a->ev[0], g->stream));
     return RKG_OK;

@@ -85,24 +85,18 @@ int refresh_speed(rkg_grid* g) {
 
 // Source terms a descriptor asks the stage kernels for.
 int feat_of(const GridDesc& d) {
-    return (d.fwx ? FEAT_FORCING : 0) | (d.sponge_width > 0 ? FEAT_SPONGE : 0);
+    bool chapman = false;
+    for (int s = 0; s < 4; ++s) chapman = chapman || d.bc[s] == RKG_BC_CHAPMAN;
+    return (d.fwx ? FEAT_FORCING : 0) | (d.sponge_width > 0 ? FEAT_SPONGE : 0) | (chapman ? FEAT_CHAPMAN : 0);
 }
 
 template <int STAGE>
 void launch_stage(rkg_grid* g, const GridDesc& d, int ntiles, int mode, double tlim) {
     const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
-    switch (feat_of(d)) {
-    case 0: launch_fused_stage<0>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf); break;
-    case FEAT_FORCING:
-        launch_fused_stage<FEAT_FORCING>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf);
-        break;
-    case FEAT_SPONGE:
-        launch_fused_stage<FEAT_SPONGE>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf);
-        break;
-    default:
-        launch_fused_stage<FEAT_FORCING | FEAT_SPONGE>(STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf);
-        break;
-    }
+    void (*const table[8])(int, hipStream_t, const GridDesc&, int, int, int, double, bool, bool) = {
+        launch_fused_stage<0>, launch_fused_stage<1>, launch_fused_stage<2>, launch_fused_stage<3>,
+        launch_fused_stage<4>, launch_fused_stage<5>, launch_fused_stage<6>, launch_fused_stage<7>};
+    table[feat_of(d)](STAGE, g->stream, d, ntiles, g->parity, mode, tlim, hllc, nf);
 }
 
 // Shu-Osher table of a generic scheme, make_timestepper
@@ -132,18 +126,10 @@ SchemeTab scheme_tab(int scheme) {
 
 void launch_gen(rkg_grid* g, const GridDesc& d, const GenStage& gs, int ntiles, int mode, double tlim) {
     const bool hllc = g->cfg.use_hllc != 0, nf = d.nfield != nullptr;
-    switch (feat_of(d)) {
-    case 0: launch_generic_stage<0>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf); break;
-    case FEAT_FORCING:
-        launch_generic_stage<FEAT_FORCING>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf);
-        break;
-    case FEAT_SPONGE:
-        launch_generic_stage<FEAT_SPONGE>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf);
-        break;
-    default:
-        launch_generic_stage<FEAT_FORCING | FEAT_SPONGE>(g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf);
-        break;
-    }
+    void (*const table[8])(hipStream_t, const GridDesc&, const GenStage&, int, int, int, double, bool, bool) = {
+        launch_generic_stage<0>, launch_generic_stage<1>, launch_generic_stage<2>, launch_generic_stage<3>,
+        launch_generic_stage<4>, launch_generic_stage<5>, launch_generic_stage<6>, launch_generic_stage<7>};
+    table[feat_of(d)](g->stream, d, gs, ntiles, g->parity, mode, tlim, hllc, nf);
 }
 
 // Generic explicit SSP steps (euler / rk3 / rk43), solver_advance_2d_generic
@@ -162,15 +148,18 @@ int launch_generic(rkg_grid* g, int n, int mode, double tlim) {
     const size_t P = g->plane;
     for (int k = 0; k < n; ++k) {
         const double* in = A;
+        const double* prev = nullptr;
         for (int st = 0; st < T.n; ++st) {
             double* out = (T.n > 1 && st == T.n - 1) ? A : ((st % 2 == 0) ? B : C);
             GenStage gs;
+            gs.ph = prev;   // the previous boundary call's state (Chapman eta_old)
             gs.ih = in; gs.ihu = in + P; gs.ihv = in + 2 * P;
             gs.oh = out; gs.ohu = out + P; gs.ohv = out + 2 * P;
             gs.alpha = T.alpha[st]; gs.beta = T.beta[st]; gs.c = T.c[st];
             gs.first = st == 0;
             gs.last = T.n > 1 && st == T.n - 1;
             launch_gen(g, d, gs, ntiles, mode, tlim);
+            prev = in;
             in = out;
         }
         if (T.n == 1) {
@@ -281,9 +270,9 @@ int check_config(const rkg_config* c) {
     for (int s = 0; s < 4; ++s) {
         switch (c->bc[s]) {
         case RKG_BC_WALL: case RKG_BC_OPEN: case RKG_BC_TIDAL: case RKG_BC_INFLOW:
-        case RKG_BC_DISCHARGE: case RKG_BC_CLAMPED: case RKG_BC_SPONGE: break;
+        case RKG_BC_DISCHARGE: case RKG_BC_CLAMPED: case RKG_BC_SPONGE: case RKG_BC_CHAPMAN: break;
         default:
-            set_error("boundary tag %d on edge %d is not implemented (nested/chapman)", c->bc[s], s);
+            set_error("boundary tag %d on edge %d is not implemented (nested)", c->bc[s], s);
             return RKG_ERR_UNSUPPORTED;
         }
     }

@@ -580,7 +580,7 @@ bool supported(const Config& c, std::string& msg) {
     if (c.use_nonhydrostatic) { msg = "use_nonhydrostatic is not implemented here"; return false; }
     if (c.use_linear_bottom_drag) { msg = "use_linear_bottom_drag is not implemented here"; return false; }
     for (int s = 0; s < 4; ++s) {
-        if (c.bc[s] == "nested" || c.bc[s] == "chapman") { msg = "boundary '" + c.bc[s] + "' is not implemented here"; return false; }
+        if (c.bc[s] == "nested") { msg = "boundary '" + c.bc[s] + "' is not implemented here"; return false; }
     }
     if (c.nghost < 2) { msg = "nghost must be >= 2 (the stencil reaches two cells)"; return false; }
     if (c.nx < 2 || c.ny < 2) { msg = "nx and ny must be >= 2 on this path"; return false; }
@@ -602,6 +602,7 @@ int bc_tag(const std::string& s) {   // bc_type_from_string, rki_boundary_types.
     if (s == "discharge") return RKG_BC_DISCHARGE;
     if (s == "clamped") return RKG_BC_CLAMPED;
     if (s == "sponge") return RKG_BC_SPONGE;
+    if (s == "chapman") return RKG_BC_CHAPMAN;
     return RKG_BC_WALL;
 }
 
@@ -797,7 +798,13 @@ double tidal_eta(const BcParams& bc, double time) {
 
 // apply_boundary_conditions on full ghosted arrays, rki_boundary.F90:40-466:
 // edges W, E, S, N in turn, W/E spanning all rows and S/N all columns.
-void apply_bc(Sim& s, double time, double* h, double* hu, double* hv, double* b) {
+// Chapman edges (:471-552) radiate on the adjacent layer of interior rows
+// / columns against eta_old = prev_h + b, the previous boundary call's
+// state, when prev_h is given and bc_dt > 0; elsewhere (the outer layer,
+// and the corner blocks, whose eta_old the library does not keep) the ghost
+// surface is the interior eta.
+void apply_bc(Sim& s, double time, double* h, double* hu, double* hv, double* b, double bc_dt = 0.0,
+              const double* prev_h = nullptr) {
     const int ng = s.ng;
     const double eta_tide = tidal_eta(s.bc, time);
     for (int side = 0; side < 4; ++side) {
@@ -851,6 +858,31 @@ void apply_bc(Sim& s, double time, double* h, double* hu, double* hv, double* b)
                     hu[G] = hu[I];
                     hv[G] = hv[I];
                     break;
+                case RKG_BC_CHAPMAN: {
+                    const double h_int = h[I];
+                    const double eta_int = h_int + b[I];
+                    const double hmax = h_int > 1.0e-6 ? h_int : 1.0e-6;
+                    const double speed = std::sqrt(GRAVITY * hmax);
+                    double eta_g = eta_int;
+                    const bool interior_t = t > ng && t <= n_tan - ng;
+                    if (kk == 1 && interior_t && prev_h && bc_dt > 0.0) {
+                        const size_t I2 = xedge ? s.ix(lo ? ng + 2 : ng + s.nx - 1, t)
+                                                : s.ix(t, lo ? ng + 2 : ng + s.ny - 1);
+                        const double dd = xedge ? s.dx : s.dy;
+                        const double eta_old = prev_h[I] + b[I];
+                        const double deta_dt = (eta_int - eta_old) / bc_dt;
+                        const double deta_dx = (h[I2] + b[I2] - eta_int) / dd;
+                        double cp = std::fabs(deta_dx) > 1.0e-10 ? -deta_dt / deta_dx : 0.0;
+                        cp = std::fmin(cp, speed);
+                        cp = 0.0 > cp ? 0.0 : cp;
+                        eta_g = eta_old - cp * bc_dt / dd * (eta_old - eta_int);
+                    }
+                    const double hg = max0(eta_g - b[G]);
+                    h[G] = hg;
+                    nrm[G] = hg * (nrm[I] / hmax + sgn * speed * (eta_int - eta_g) / hmax);
+                    tan[G] = tan[I];
+                    break;
+                }
                 default: {   // inflow, discharge
                     const double in_sgn = -sgn;
                     const double h_int = h[I];
@@ -899,7 +931,11 @@ int copy_back(Sim& s) {
                 uhu[k] = s.hu[k];
                 uhv[k] = s.hv[k];
             }
-        apply_bc(s, s.t_current, uh.data(), uhu.data(), uhv.data(), s.b.data());
+        double t_now, dt_last;
+        long long n_steps;
+        if ((rc = rkg_rc(rkg_get_clock(s.dev, &t_now, &dt_last, &n_steps), "rkg_get_clock"))) return rc;
+        // stage 2's call: U* at the new time, eta_old from U^n (the RK save)
+        apply_bc(s, s.t_current, uh.data(), uhu.data(), uhv.data(), s.b.data(), dt_last, s.h0.data());
         for (int j = 1; j <= s.nyt; ++j)
             for (int i = 1; i <= s.nxt; ++i) {
                 const bool in = i > s.ng && i <= s.ng + s.nx && j > s.ng && j <= s.ng + s.ny;

@@ -69,6 +69,7 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     StageCtx s;
     s.alpha = 0.0; s.a_prev = 1.0;
     s.blend = false; s.first = false; s.last = false;
+    s.radiate = false; s.prev_h = nullptr; s.bc_dt = 0.0;
     if (STAGE == 1 || (STAGE == 3 && gs.first)) {
         const CtrlSlot cp = d.ctrl[p];
         const bool active = (mode == MODE_STEP) || (cp.t < tlim - 1.0e-10);
@@ -90,6 +91,7 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
             }
         }
         if (!active) return;
+        s.bc_dt = dt;   // bc%dt = state%dt, rki_solver.F90:748
         if (STAGE == 1) {
             s.ih = d.ah; s.ihu = d.ahu; s.ihv = d.ahv;
             s.oh = d.bh; s.ohu = d.bhu; s.ohv = d.bhv;
@@ -108,6 +110,9 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
         s.oh = gs.oh; s.ohu = gs.ohu; s.ohv = gs.ohv;
         s.dt = gs.beta * cq.dt;
         s.eta_tidal = tidal_eta(d, cp.t + gs.c * cq.dt);  // bc%time = t_base + c_i dt
+        s.bc_dt = cq.dt;
+        s.radiate = true;       // against the previous stage's input
+        s.prev_h = gs.ph;
     } else {
         const CtrlSlot cq = d.ctrl[1 - p];
         if (!cq.active) return;
@@ -115,6 +120,9 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
         s.oh = d.o2h; s.ohu = d.o2hu; s.ohv = d.o2hv;
         s.dt = cq.dt;
         s.eta_tidal = tidal_eta(d, cq.t);  // bc%time = t + dt
+        s.bc_dt = cq.dt;
+        s.radiate = true;       // against U^n, the input of stage 1
+        s.prev_h = d.ah;
     }
     if (STAGE == 3) {
         s.alpha = gs.alpha;
@@ -127,6 +135,8 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     s.rdx = recip(d.dx);
     s.rdy = recip(d.dy);
     const int strip = w.x, seg = w.y;
+    s.row0 = seg * d.ty;
+    s.row1 = min(s.row0 + d.ty, d.ny);
     double smax, fsum, ssum;
     if (tile_is_interior(d, strip, seg))
         smax = process_tile<STAGE, HLLC, false, NF, REC, FEAT>(d, s, L, strip, seg, fsum, ssum);

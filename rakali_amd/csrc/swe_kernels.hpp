@@ -34,6 +34,7 @@ struct alignas(64) CtrlSlot {
 // Optional source terms of the stage kernels (template bit mask).
 constexpr int FEAT_FORCING = 1;   // surface forcing, rki_kernel_fused.F90:268-289
 constexpr int FEAT_SPONGE = 2;    // sponge relaxation, rki_kernel_sponge.F90:16-88
+constexpr int FEAT_CHAPMAN = 4;   // Chapman radiation edges, rki_boundary.F90:471-552
 
 struct GridDesc {
     int nx, ny;
@@ -243,6 +244,14 @@ struct StageCtx {
     double eta_tidal;
     Recip rdx, rdy;
     SourceTerms src;
+    // Chapman edges (FEAT_CHAPMAN): whether this boundary call radiates
+    // (its state differs from the previous call's), the previous call's
+    // state prev_h (its eta_old = prev_h + b at the first interior cell),
+    // the step's dt (bc%dt) and the tile's own rows
+    bool radiate;
+    const double* prev_h;
+    double bc_dt;
+    int row0, row1;
     // generic SSP stage only (STAGE 3)
     double alpha, a_prev;         // blend weights on U^n and on the Euler iterate
     bool blend, first, last;
@@ -255,12 +264,73 @@ struct StageCtx {
 struct GenStage {
     const double* ih; const double* ihu; const double* ihv;
     double* oh; double* ohu; double* ohv;
+    const double* ph;             // h of the previous stage's input (Chapman eta_old), or null
     double alpha, beta, c;        // rki_timestepper.F90:53-58
     int first, last;
 };
 
-template <bool EDGE>
+// Ghost cell (c, j) of a Chapman edge (bc_chapman_edge, rki_boundary.F90:
+// 471-552).  The reference calls the boundary before the dt computation
+// (rki_solver.F90:718-720) and before each stage; each call sets eta_old
+// to the interior eta of the state it sees (on the k = 1 layer, before the
+// k = 2 layer of the same t reads it: t outer, k inner, the serial order of
+// the reference's CPU build).  So
+// * the first stage of a step sees eta_old = its own interior eta: the
+//   ghost surface is the interior eta (the radiation term vanishes);
+// * a later stage radiates on the adjacent layer against the previous
+//   call's state, U^n for SSP-RK2 stage 2, the previous stage's input for
+//   the generic schemes, with the step's dt; the outer layer again sees
+//   eta_old = its interior eta.
+// The bed ghost is the first interior bed (fill_bathymetry_ghosts_array).
+// x-edge ghosts of rows outside the tile's own are never used (the y
+// pipeline loads them for lanes past the grid); they skip the radiation,
+// whose eta_old the owning tile may be overwriting in place.
+__device__ __forceinline__ Cell chapman_ghost(const GridDesc& d, const StageCtx& s, int side, int c, int j) {
+    const bool xedge = side < 2, lo = (side & 1) == 0;
+    const int kk = xedge ? (lo ? -c - 1 : c - d.nx) : (lo ? -j - 1 : j - d.ny);
+    Cell g;
+    g.h = 0.0; g.hu = 0.0; g.hv = 0.0; g.b = 0.0;
+    if (kk > 1) return g;
+    int ci = c, ji = j, c2 = c, j2 = j;
+    if (xedge) { ci = lo ? 0 : d.nx - 1; c2 = lo ? 1 : d.nx - 2; }
+    else { ji = lo ? 0 : d.ny - 1; j2 = lo ? 1 : d.ny - 2; }
+    const Cell in0 = load_direct(s.ih, s.ihu, s.ihv, d.bed, d.pitch, ci, ji);
+    const double h_int = in0.h;
+    const double eta_int = h_int + in0.b;
+    const double hmax = dmax(h_int, 1.0e-6);
+    const double speed = sqrt(d.g * hmax);
+    double eta_g = eta_int;
+    const bool own_row = !xedge || (j >= s.row0 && j < s.row1);
+    if (kk == 0 && s.radiate && s.bc_dt > 0.0 && own_row) {
+        const long long k0 = (long long)ji * d.pitch + ci;
+        const long long k2 = (long long)j2 * d.pitch + c2;
+        const double dd = xedge ? d.dx : d.dy;
+        const double eta_old = s.prev_h[k0] + d.bed[k0];
+        const double deta_dt = (eta_int - eta_old) / s.bc_dt;
+        const double deta_dx = (s.ih[k2] + d.bed[k2] - eta_int) / dd;
+        double cp = fabs(deta_dx) > 1.0e-10 ? -deta_dt / deta_dx : 0.0;
+        cp = dmax(0.0, fmin(cp, speed));
+        eta_g = eta_old - cp * s.bc_dt / dd * (eta_old - eta_int);
+    }
+    g.b = in0.b;
+    g.h = fmax0(eta_g - g.b);
+    const double sgn = lo ? -1.0 : 1.0;
+    const double nrm = xedge ? in0.hu : in0.hv;
+    const double rad = g.h * (nrm / hmax + sgn * speed * (eta_int - eta_g) / hmax);
+    if (xedge) { g.hu = rad; g.hv = in0.hv; }
+    else { g.hv = rad; g.hu = in0.hu; }
+    return g;
+}
+
+template <bool EDGE, int FEAT = 0>
 __device__ __forceinline__ Cell ld(const GridDesc& d, const StageCtx& s, int c, int j) {
+    if constexpr (EDGE && (FEAT & FEAT_CHAPMAN) != 0) {
+        const bool xin = (unsigned)c < (unsigned)d.nx, yin = (unsigned)j < (unsigned)d.ny;
+        if (xin != yin) {
+            const int side = !xin ? (c < 0 ? 0 : 1) : (j < 0 ? 2 : 3);
+            if (d.bc[side] == BC_CHAPMAN) return chapman_ghost(d, s, side, c, j);
+        }
+    }
     return load_cell<EDGE>(d, s.ih, s.ihu, s.ihv, c, j, s.eta_tidal);
 }
 
@@ -329,20 +399,20 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     bool edge_tiny = false;
     if (lane < nrows) {
         const int j = j0 + lane;
-        const Cell a = ld<EDGE>(d, s, c0 - 2, j);
-        const Cell bl = ld<EDGE>(d, s, c0 - 1, j);
-        const Cell cc = ld<EDGE>(d, s, c0, j);
+        const Cell a = ld<EDGE, FEAT>(d, s, c0 - 2, j);
+        const Cell bl = ld<EDGE, FEAT>(d, s, c0 - 1, j);
+        const Cell cc = ld<EDGE, FEAT>(d, s, c0, j);
         FState lo, hi;
         cell_states(a, bl, cc, lo, hi);
         L.hl[0][lane] = bl.h; L.hl[1][lane] = bl.hu; L.hl[2][lane] = bl.hv; L.hl[3][lane] = bl.b;
         L.ph[0][lane] = hi.h; L.ph[1][lane] = hi.hu; L.ph[2][lane] = hi.hv;
-        const Cell m1 = ld<EDGE>(d, s, c0 + 63, j);
-        const Cell p0 = ld<EDGE>(d, s, c0 + 64, j);
+        const Cell m1 = ld<EDGE, FEAT>(d, s, c0 + 63, j);
+        const Cell p0 = ld<EDGE, FEAT>(d, s, c0 + 64, j);
         L.hr[0][lane] = p0.h; L.hr[1][lane] = p0.hu; L.hr[2][lane] = p0.hv; L.hr[3][lane] = p0.b;
         edge_tiny = tiny(a) || tiny(bl) || tiny(cc) || tiny(m1) || tiny(p0);
         if (c0 + WAVE - 1 < d.nx) {
-            const Cell m2 = ld<EDGE>(d, s, c0 + 62, j);
-            const Cell p1 = ld<EDGE>(d, s, c0 + 65, j);
+            const Cell m2 = ld<EDGE, FEAT>(d, s, c0 + 62, j);
+            const Cell p1 = ld<EDGE, FEAT>(d, s, c0 + 65, j);
             FState lo1, hi1, lo2, hi2;
             cell_states(m2, m1, p0, lo1, hi1);
             cell_states(m1, p0, p1, lo2, hi2);
@@ -360,11 +430,11 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     Face fyb;
     bool t_prev, t_rc, t_rn, t_rnn;
     {
-        const Cell a = ld<EDGE>(d, s, c, j0 - 2);
-        const Cell bm = ld<EDGE>(d, s, c, j0 - 1);
-        rc = ld<EDGE>(d, s, c, j0);
-        rn = ld<EDGE>(d, s, c, j0 + 1);
-        rnn = ld<EDGE>(d, s, c, j0 + 2);
+        const Cell a = ld<EDGE, FEAT>(d, s, c, j0 - 2);
+        const Cell bm = ld<EDGE, FEAT>(d, s, c, j0 - 1);
+        rc = ld<EDGE, FEAT>(d, s, c, j0);
+        rn = ld<EDGE, FEAT>(d, s, c, j0 + 1);
+        rnn = ld<EDGE, FEAT>(d, s, c, j0 + 2);
         FState lo, tm;
         cell_states(a, bm, rc, lo, tm);
         FState bc;
@@ -389,7 +459,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         const bool more = j + 1 < j1;
         Cell nxt, nun;
         if (more) {
-            nxt = ld<EDGE>(d, s, c, j + 3);
+            nxt = ld<EDGE, FEAT>(d, s, c, j + 3);
             if (read_un && own) {
                 const long long k = (long long)(j + 1) * pitch + c;
                 nun.h = d.ah[k]; nun.hu = d.ahu[k]; nun.hv = d.ahv[k];

@@ -11,8 +11,8 @@ garbage collection).  The library it binds is ``rakali_amd/librakali.so``
 fallback, and a missing library raises ``FileNotFoundError``.
 
 Differences, all from features this build does not run: multilayer and
-non-hydrostatic simulations, nested and Chapman edges and the
-semi-implicit scheme raise ``RuntimeError`` (the library returns -5);
+non-hydrostatic simulations, nested edges and the semi-implicit scheme
+raise ``RuntimeError`` (the library returns -5);
 ``save_restart`` writes the library's binary restart format instead of
 NetCDF; ``save_checkpoint`` writes classic NetCDF with scipy;
 ``to_xarray`` needs xarray.

@@ -49,7 +49,7 @@ def test_bad_handles_and_arguments():
     out = ctypes.c_void_p()
     assert L.rkg_create(ctypes.byref(c), None, ctypes.byref(out)) == -2
     c.nx = 8
-    c.bc[0] = 9   # chapman: not implemented
+    c.bc[0] = 4   # nested: not implemented
     assert L.rkg_create(ctypes.byref(c), None, ctypes.byref(out)) == -5
     assert out.value is None
 

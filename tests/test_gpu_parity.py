@@ -233,7 +233,7 @@ def test_errors_and_lifecycle():
     with pytest.raises(RkgError):
         Grid(1, 5, 1.0, 1.0)
     with pytest.raises(ValueError):
-        Grid(4, 4, 1.0, 1.0, bc=("chapman", "wall", "wall", "wall"))
+        Grid(4, 4, 1.0, 1.0, bc=("nested", "wall", "wall", "wall"))
     g = Grid(8, 8, 1.0, 1.0)
     with pytest.raises(ValueError):
         g.set_field("h", np.zeros((7, 8)))

@@ -95,7 +95,6 @@ def test_namelist_syntax_forms(L):
     ("&nonhydrostatic_nml\n use_multilayer = .true.\n/", -5, "use_multilayer"),
     ("&nonhydrostatic_nml\n use_nonhydrostatic = .true.\n/", -5, "nonhydrostatic"),
     ("&time_nml\n time_scheme = 'semi_implicit'\n/", -5, "semi_implicit"),
-    ("&boundary_nml\n bc_north = 'chapman'\n/", -5, "chapman"),
     ("&grid_nml\n nghost = 1\n/", -5, "nghost"),
     ("&grid_nml\n nx = 8, ny = 1\n/", -5, "ny must be"),
 ])
