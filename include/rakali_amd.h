@@ -86,6 +86,14 @@ extern "C" {
 #define RKG_FIELD_WIND_Y 12
 #define RKG_FIELD_PRESSURE 13
 #define RKG_FIELD_PRECIP 14
+/* Read-only: the state of the last step's last boundary call (U* of
+ * SSP-RK2 stage 2, the last stage's input of a generic scheme, U^n for
+ * euler) and h of the boundary call before it, for rebuilding the ghost
+ * cells the reference leaves after a step.  Need rkg_set_rk_save on. */
+#define RKG_FIELD_BC_H 15
+#define RKG_FIELD_BC_HU 16
+#define RKG_FIELD_BC_HV 17
+#define RKG_FIELD_BC_PREV_H 18
 
 #define RKG_MAX_TIDAL_CONSTITUENTS 10
 
@@ -198,6 +206,9 @@ int rkg_get_mass_budget(rkg_grid *g, double *mass_out);
  * 273-289).  The first call allocates the four RKG_FIELD_WIND_X..PRECIP
  * planes (32 B per cell); they stay until rkg_destroy.  The adjoint does
  * not support forcing. */
+/* bc%time and bc%dt of the last step's last boundary call (stage 2 at
+ * t + dt for SSP-RK2, t + c_last dt for a generic scheme). */
+int rkg_get_last_bc(rkg_grid *g, double *time, double *dt);
 int rkg_set_forcing(rkg_grid *g, int active);
 int rkg_get_forcing(rkg_grid *g, int *active);
 /* Surface mass source state%barotropic%mass_src (accumulate_mass_src,

@@ -94,6 +94,7 @@ SIGNATURES = {
     "rkg_get_time_scheme": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     "rkg_set_mass_budget": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_get_mass_budget": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "rkg_get_last_bc": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "rkg_set_forcing": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_get_forcing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     "rkg_get_mass_source": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),

@@ -156,6 +156,7 @@ int launch_generic(rkg_grid* g, int n, int mode, double tlim) {
             gs.ih = in; gs.ihu = in + P; gs.ihv = in + 2 * P;
             gs.oh = out; gs.ohu = out + P; gs.ohv = out + 2 * P;
             gs.alpha = T.alpha[st]; gs.beta = T.beta[st]; gs.c = T.c[st];
+            gs.c_last = T.c[T.n - 1];
             gs.first = st == 0;
             gs.last = T.n > 1 && st == T.n - 1;
             launch_gen(g, d, gs, ntiles, mode, tlim);
@@ -371,7 +372,10 @@ static int copy_field(rkg_grid* g, int field, double* host_or_dev, int ld, bool 
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     double* f = field_ptr(g, field);
     if (!f) { set_error("unknown or unallocated field %d", field); return RKG_ERR_RANGE; }
-    if (upload && field > RKG_FIELD_B && field < RKG_FIELD_N) { set_error("field %d is read-only", field); return RKG_ERR_RANGE; }
+    if (upload && ((field > RKG_FIELD_B && field < RKG_FIELD_N) || field >= RKG_FIELD_BC_H)) {
+        set_error("field %d is read-only", field);
+        return RKG_ERR_RANGE;
+    }
     if (!host_or_dev || ld < g->cfg.nx) { set_error("bad buffer or leading dimension"); return RKG_ERR_RANGE; }
     const size_t w = (size_t)g->cfg.nx * sizeof(double);
     if (upload) {
@@ -577,6 +581,17 @@ int rkg_get_forcing(rkg_grid* g, int* active) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     if (!active) return RKG_ERR_RANGE;
     *active = g->forcing ? 1 : 0;
+    return RKG_OK;
+}
+
+int rkg_get_last_bc(rkg_grid* g, double* time, double* dt) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (!time || !dt) return RKG_ERR_RANGE;
+    CtrlSlot c;
+    int rc = read_clock(g, &c);
+    if (rc) return rc;
+    *time = c.t_bc;
+    *dt = c.dt;
     return RKG_OK;
 }
 

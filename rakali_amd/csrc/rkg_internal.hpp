@@ -93,6 +93,26 @@ inline double* field_ptr(rkg_grid* g, int field) {
     case RKG_FIELD_WIND_Y: return g->frc ? g->frc + g->plane : nullptr;
     case RKG_FIELD_PRESSURE: return g->frc ? g->frc + 2 * g->plane : nullptr;
     case RKG_FIELD_PRECIP: return g->frc ? g->frc + 3 * g->plane : nullptr;
+    case RKG_FIELD_BC_H:
+    case RKG_FIELD_BC_HU:
+    case RKG_FIELD_BC_HV:
+    case RKG_FIELD_BC_PREV_H: {
+        // the last step's last boundary call: its state and the previous
+        // call's (rki_solver.F90:760-815 for SSP-RK2, :861-925 for the
+        // generic stage loop); stage buffers B = planes 3-5, C = cbuf
+        double* B = plane_ptr(g, 3);
+        double* cur = nullptr;
+        double* prev = nullptr;
+        switch (g->scheme) {
+        case RKG_TIME_EULER: cur = g->save; prev = g->save; break;   // U^n, after the pre-dt call on U^n
+        case RKG_TIME_RK3: cur = g->cbuf; prev = B; break;           // stage 3 input, stage 2 input
+        case RKG_TIME_RK43: cur = B; prev = g->cbuf; break;          // stage 4 input, stage 3 input
+        default: cur = B; prev = g->save; break;                     // U*, U^n
+        }
+        if (!cur || !prev) return nullptr;
+        if (field == RKG_FIELD_BC_PREV_H) return prev;
+        return cur + (size_t)(field - RKG_FIELD_BC_H) * g->plane;
+    }
     default: return nullptr;
     }
 }

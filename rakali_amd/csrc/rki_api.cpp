@@ -904,8 +904,10 @@ int download_interior(Sim& s, int field, std::vector<double>& dst) {
 }
 
 // Device -> host copy of the state.  After a step the reference's ghost
-// cells hold the last boundary call, made on U* at the new time
-// (rki_solver.F90:790-795); they are rebuilt here from U*.
+// cells hold the step's last boundary call: on U* at the new time for
+// SSP-RK2 (rki_solver.F90:790-795), on the last stage's input at
+// t + c_last dt for the generic schemes (:895-897); they are rebuilt here
+// from that state.
 int copy_back(Sim& s) {
     int rc;
     if ((rc = download_interior(s, RKG_FIELD_H, s.h))) return rc;
@@ -917,9 +919,11 @@ int copy_back(Sim& s) {
     if (s.stepped) {
         const size_t n = s.h.size();
         std::vector<double> uh(n, 0.0), uhu(n, 0.0), uhv(n, 0.0);
-        if ((rc = download_interior(s, RKG_FIELD_HS, uh))) return rc;
-        if ((rc = download_interior(s, RKG_FIELD_HUS, uhu))) return rc;
-        if ((rc = download_interior(s, RKG_FIELD_HVS, uhv))) return rc;
+        std::vector<double> prev(n, 0.0);
+        if ((rc = download_interior(s, RKG_FIELD_BC_H, uh))) return rc;
+        if ((rc = download_interior(s, RKG_FIELD_BC_HU, uhu))) return rc;
+        if ((rc = download_interior(s, RKG_FIELD_BC_HV, uhv))) return rc;
+        if ((rc = download_interior(s, RKG_FIELD_BC_PREV_H, prev))) return rc;
         // ghosts of U* from the current ghosts of h, hu, hv (corner
         // blocks read them), then the boundary call at the new time
         for (int j = 1; j <= s.nyt; ++j)
@@ -931,11 +935,11 @@ int copy_back(Sim& s) {
                 uhu[k] = s.hu[k];
                 uhv[k] = s.hv[k];
             }
-        double t_now, dt_last;
-        long long n_steps;
-        if ((rc = rkg_rc(rkg_get_clock(s.dev, &t_now, &dt_last, &n_steps), "rkg_get_clock"))) return rc;
-        // stage 2's call: U* at the new time, eta_old from U^n (the RK save)
-        apply_bc(s, s.t_current, uh.data(), uhu.data(), uhv.data(), s.b.data(), dt_last, s.h0.data());
+        double t_bc, dt_last;
+        if ((rc = rkg_rc(rkg_get_last_bc(s.dev, &t_bc, &dt_last), "rkg_get_last_bc"))) return rc;
+        // the last boundary call of the step: its state, bc%time and bc%dt,
+        // and the previous call's state (Chapman eta_old)
+        apply_bc(s, t_bc, uh.data(), uhu.data(), uhv.data(), s.b.data(), dt_last, prev.data());
         for (int j = 1; j <= s.nyt; ++j)
             for (int i = 1; i <= s.nxt; ++i) {
                 const bool in = i > s.ng && i <= s.ng + s.nx && j > s.ng && j <= s.ng + s.ny;

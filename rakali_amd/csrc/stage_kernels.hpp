@@ -84,6 +84,11 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
             q->mass_out = cp.mass_out;
             q->mass_src = cp.mass_src;
             q->active = active ? 1 : 0;
+            // the last boundary call: stage 2 at t + dt, or the last generic
+            // stage at t + c_last dt (rki_solver.F90:790-795, :895-897)
+            if (!active) q->t_bc = cp.t_bc;
+            else if (STAGE == 1) q->t_bc = cp.t + dt;
+            else q->t_bc = cp.t + gs.c_last * dt;
             if (REC && d.hist != nullptr) {   // the step's clock for a reverse sweep (rki_solver.F90:731)
                 d.hist[0] = cp.t;
                 d.hist[1] = dt;
@@ -193,6 +198,7 @@ __global__ void k_ctrl_reset(CtrlSlot* ctrl, int p, double t, double dt, long lo
         ctrl[p].mass_out = mass_out;
         ctrl[p].mass_src = mass_src;
         ctrl[p].active = 1;
+        ctrl[p].t_bc = t;
     }
 }
 

@@ -29,6 +29,7 @@ struct alignas(64) CtrlSlot {
     int active;                   // 1 if the step that wrote this slot ran
     int pad;
     double mass_src;              // surface mass source (accumulate_mass_src, rki_solver.F90:539-606)
+    double t_bc;                  // bc%time of the step's last boundary call
 };
 
 // Optional source terms of the stage kernels (template bit mask).
@@ -266,6 +267,7 @@ struct GenStage {
     double* oh; double* ohu; double* ohv;
     const double* ph;             // h of the previous stage's input (Chapman eta_old), or null
     double alpha, beta, c;        // rki_timestepper.F90:53-58
+    double c_last;                // c of the scheme's last stage (bc%time of its last call)
     int first, last;
 };
 

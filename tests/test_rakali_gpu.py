@@ -31,7 +31,7 @@ def rakali_from_scenario(scn, ng=3, **override):
     args.update({f"bc_{s}": v for s, v in zip(SIDES, bc)})
     for k in ("cfl", "manning_n", "use_hllc", "t_end", "tidal_amplitude", "tidal_period",
               "coriolis_f", "dt_fixed", "cfl_interval", "inflow_velocity", "discharge_flux",
-              "clamped_elevation"):
+              "clamped_elevation", "time_scheme"):
         if k in kw:
             args[k] = kw.pop(k)
     sim = Rakali(defer_init=True, **args)
@@ -459,3 +459,30 @@ def test_save_checkpoint_netcdf(tmp_path):
             assert (nc.nx_total, nc.ny_total, nc.nghost) == (30 + 2 * ng, 20 + 2 * ng, ng)
             assert nc.manning_n == 0.02 and nc.coriolis_f == 1e-4 and nc.dx == 5.0 and nc.dy == 4.0
             assert nc.variables["b"].positive == b"up"
+
+
+@pytest.mark.parametrize("scheme, bc", [
+    ("euler", ("tidal", "wall", "open", "wall")),
+    ("rk3", ("tidal", "open", "clamped", "wall")),
+    ("rk43", ("inflow", "tidal", "wall", "discharge")),
+    ("rk3", ("chapman", "chapman", "wall", "tidal")),
+])
+def test_generic_scheme_full_arrays_match_oracle(scheme, bc):
+    """The handle API with time_scheme euler / rk3 / rk43: the interior and
+    the ghost rings, which hold the step's last boundary call (the last
+    stage's input at t + c_last dt, rki_solver.F90:861-925), against the
+    oracle; the RK save is U^n (:879-881)."""
+    from rakali_amd import scenarios as S
+    codes = {"euler": 1, "rk3": 3, "rk43": 4}
+    scn = S.wetdry_islands(nx=40, ny=28, steps=15, bc=bc)
+    extra = dict(tidal_amplitude=0.6, tidal_period=400.0, inflow_velocity=0.3, discharge_flux=0.5,
+                 clamped_elevation=0.2)
+    print(f"[rki generic] {scheme} bc={bc}", flush=True)
+    with rakali_from_scenario(scn, time_scheme=scheme, **extra) as sim:
+        sim.step(scn.steps)
+        osim, _ = run_oracle(scn, ng=3, time_scheme=codes[scheme], **extra)
+        assert sim.time == osim.t and sim.dt == osim.dt
+        assert_full_equal(sim, osim)
+        _, prev = run_oracle(scn, steps=scn.steps - 1, ng=3, time_scheme=codes[scheme], **extra)
+        for k in ("h", "hu", "hv"):
+            assert np.array_equal(getattr(sim, k + "0")[3:-3, 3:-3], prev[k]), k
