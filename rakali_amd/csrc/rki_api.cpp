@@ -578,7 +578,8 @@ bool supported(const Config& c, std::string& msg) {
     if (c.time_scheme == "semi_implicit") { msg = "time_scheme 'semi_implicit' is not implemented here"; return false; }
     if (c.use_multilayer) { msg = "use_multilayer is not implemented here"; return false; }
     if (c.use_nonhydrostatic) { msg = "use_nonhydrostatic is not implemented here"; return false; }
-    if (c.use_linear_bottom_drag) { msg = "use_linear_bottom_drag is not implemented here"; return false; }
+    // use_linear_bottom_drag / bottom_drag_r: read only by the multilayer
+    // step (rki_solver.F90:1393,1522,1684); the 2-D step ignores them
     for (int s = 0; s < 4; ++s) {
         if (c.bc[s] == "nested") { msg = "boundary '" + c.bc[s] + "' is not implemented here"; return false; }
     }

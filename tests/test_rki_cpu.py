@@ -105,6 +105,16 @@ def test_namelist_errors(L, text, code, pattern):
     assert re.search(pattern, L.rki_last_error().decode()), L.rki_last_error()
 
 
+def test_two_d_step_accepts_multilayer_only_physics(L):
+    # use_linear_bottom_drag / bottom_drag_r / wind_surface_layer are read by
+    # the multilayer step only (rki_solver.F90:1393,1522,1557): accepted and
+    # without effect here, as in the reference's 2-D step
+    rc, h = create(L, "&grid_nml\n nx=8, ny=8\n/\n&physics_nml\n use_linear_bottom_drag = .true.\n"
+                      " bottom_drag_r = 0.003\n wind_surface_layer = .true.\n/")
+    assert rc == 0
+    destroy(L, h)
+
+
 def test_time_unit_scales_t_end(L):
     # t_end = 2 hr -> 7200 s: a phase-1 handle cannot step, but rki_step
     # before finalize reports -1 and the parse itself must succeed
