@@ -29,7 +29,8 @@ HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp",
            CSRC / "rkg_internal.hpp", ROOT / "include" / "rakali_amd.h"]
 RKI_LIB = PKG / "librakali.so"
 RKI_SOURCES = [CSRC / "rki_api.cpp"]
-RKI_HEADERS = [CSRC / "crmath.hpp", ROOT / "include" / "rakali.h", ROOT / "include" / "rakali_amd.h"]
+RKI_HEADERS = [CSRC / "crmath.hpp", CSRC / "nc_classic.hpp", ROOT / "include" / "rakali.h",
+               ROOT / "include" / "rakali_amd.h"]
 DRIVER = PKG / "rakali_driver"
 DRIVER_SOURCES = [PKG / "fortran" / "rakali_amd_api.F90", PKG / "fortran" / "rakali_driver.F90"]
 ARCH = "gfx950"

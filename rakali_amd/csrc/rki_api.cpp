@@ -15,6 +15,7 @@
 //     (apply_boundary_conditions on U* at t + dt, rki_solver.F90:790-795),
 //     recomputed on the host when the state is copied back.
 // The step itself runs on the GPU through rkg_step / rkg_advance.
+#include "nc_classic.hpp"
 #include "rakali.h"
 #include "rakali_amd.h"
 #include "crmath.hpp"
@@ -1457,24 +1458,18 @@ int rki_save_restart(void* handle, const char* filename, int len) {
     int rc = refresh_host(*s);
     if (rc) return rc;
     const std::string path(filename, (size_t)len);
-    FILE* f = std::fopen(path.c_str(), "wb");
-    if (!f) {
-        set_error("cannot open '%s' for writing", path.c_str());
-        return -3;
-    }
-    const int32_t hdr[4] = {RESTART_VERSION, s->nx, s->ny, s->ng};
-    double dt = rki_get_dt(handle);
-    const double times[2] = {s->t_current, dt};
-    bool ok = std::fwrite(RESTART_MAGIC, 1, 4, f) == 4 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
-              std::fwrite(times, sizeof times, 1, f) == 1;
-    std::vector<double> row((size_t)s->nx);
-    for (const std::vector<double>* fld : {&s->h, &s->hu, &s->hv, &s->b})
-        for (int j = 1; ok && j <= s->ny; ++j)
-            ok = std::fwrite(&(*fld)[s->ix(s->ng + 1, s->ng + j)], sizeof(double), (size_t)s->nx, f) ==
-                 (size_t)s->nx;
-    ok = (std::fclose(f) == 0) && ok;
-    if (!ok) {
-        set_error("short write to '%s'", path.c_str());
+    // restart_write, rki_output.F90:740-848: interior h, hu, hv, b and time
+    std::vector<rk::nc::Field2D> f = {
+        {"h", "m", s->interior(s->h), s->nxt},
+        {"hu", "m2 s-1", s->interior(s->hu), s->nxt},
+        {"hv", "m2 s-1", s->interior(s->hv), s->nxt},
+        {"b", "m", s->interior(s->b), s->nxt},
+    };
+    char dx[64], dy[64];
+    std::snprintf(dx, sizeof dx, "%.17g", s->dx);
+    std::snprintf(dy, sizeof dy, "%.17g", s->dy);
+    if (!rk::nc::write_restart(path, s->nx, s->ny, f, s->t_current, dx, dy, std::to_string(s->ng))) {
+        set_error("cannot write restart file '%s'", path.c_str());
         return -3;
     }
     return 0;
@@ -1487,6 +1482,45 @@ int rki_load_restart(void* handle, const char* filename, int len) {
     if (s->is_finalised) return -2;
     if (!filename || len < 0) return -2;
     const std::string path(filename, (size_t)len);
+    {   // classic NetCDF (restart_read, rki_output.F90:850-988)
+        rk::nc::Reader r;
+        if (rk::nc::load_file(path, r.buf) && r.buf.size() >= 3 && std::memcmp(r.buf.data(), "CDF", 3) == 0) {
+            if (!r.parse()) {
+                set_error("restart file '%s' is not a readable classic NetCDF file", path.c_str());
+                return -3;
+            }
+            if ((int)r.dim_len("x") != s->nx || (int)r.dim_len("y") != s->ny) {
+                set_error("restart file '%s' holds a %ux%u grid, the configuration %dx%d", path.c_str(),
+                          r.dim_len("x"), r.dim_len("y"), s->nx, s->ny);
+                return -3;
+            }
+            const size_t n = (size_t)s->nx * (size_t)s->ny;
+            std::vector<double> buf(4 * n);
+            const char* names[4] = {"h", "hu", "hv", "b"};
+            double t = 0.0;
+            for (int k = 0; k < 4; ++k) {
+                const rk::nc::Var* v = r.find(names[k]);
+                if (!v || v->dims.size() != 2 || !r.read_doubles(*v, n, buf.data() + k * n)) {
+                    set_error("restart file '%s': variable %s missing or not FP64 (y, x)", path.c_str(), names[k]);
+                    return -3;
+                }
+            }
+            const rk::nc::Var* vt = r.find("time");
+            if (!vt || !r.read_doubles(*vt, 1, &t)) {
+                set_error("restart file '%s': no FP64 time", path.c_str());
+                return -3;
+            }
+            set_interior(*s, s->h, buf.data(), s->nx, s->ny);
+            set_interior(*s, s->hu, buf.data() + n, s->nx, s->ny);
+            set_interior(*s, s->hv, buf.data() + 2 * n, s->nx, s->ny);
+            set_interior(*s, s->b, buf.data() + 3 * n, s->nx, s->ny);
+            s->t_current = t;
+            s->user_set_h = true;
+            s->user_set_bathymetry = true;
+            return 0;
+        }
+    }
+    // the binary format of earlier versions of this library
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) {
         set_error("cannot open restart file '%s'", path.c_str());

@@ -267,6 +267,15 @@ def test_restart_round_trip(tmp_path):
         a.save_restart(ckpt)
         a.advance(9.0)
         h_cont = a.h_phys.copy()
+    # a classic NetCDF file with restart_write's layout (rki_output.F90:
+    # 740-848): x, y dimensions, h, hu, hv, b on (y, x), a scalar time
+    from scipy.io import netcdf_file
+    with netcdf_file(ckpt, "r", mmap=False) as nc:
+        assert nc.dimensions["x"] == NX and nc.dimensions["y"] == NY
+        assert nc.variables["h"].dimensions == ("y", "x")
+        assert np.array_equal(nc.variables["h"][:].T, h_saved)
+        assert float(nc.variables["time"].getValue()) == t_saved
+        assert nc.variables["hu"].units == b"m2 s-1" and nc.Conventions == b"CF-1.11"
     with Rakali.from_restart(ckpt, **cfg) as b:
         assert b.time == t_saved
         assert np.array_equal(b.h_phys, h_saved)

@@ -215,7 +215,32 @@ def test_load_restart_phase1(L, tmp_path):
         assert getattr(L, getter)(h, ctypes.byref(ptr), ctypes.byref(nxt), ctypes.byref(nyt)) == 0
         full = np.ctypeslib.as_array((ctypes.c_double * (11 * 9)).from_address(ptr.value)).reshape(11, 9, order="F")
         assert np.array_equal(full[3:8, 3:6], want)
+    # the classic NetCDF restart (restart_write's variables, rki_output.F90:
+    # 740-848), here written by scipy: x fastest, dimensions (y, x)
+    from scipy.io import netcdf_file
+    nc_fields = [rng.random((5, 3)) for _ in range(4)]
+    pn = tmp_path / "r.nc"
+    with netcdf_file(str(pn), "w", version=2) as nc:
+        nc.createDimension("x", 5)
+        nc.createDimension("y", 3)
+        for name, arr in zip(("h", "hu", "hv", "b"), nc_fields):
+            v = nc.createVariable(name, "d", ("y", "x"))
+            v[:, :] = arr.T
+        t = nc.createVariable("time", "d", ())
+        t[...] = 99.25
+    pnb = str(pn).encode()
+    assert L.rki_load_restart(h, pnb, len(pnb)) == 0
+    assert L.rki_get_time(h) == 99.25
+    for getter, want in zip(("rki_get_h_ptr", "rki_get_hu_ptr", "rki_get_hv_ptr", "rki_get_b_ptr"), nc_fields):
+        assert getattr(L, getter)(h, ctypes.byref(ptr), ctypes.byref(nxt), ctypes.byref(nyt)) == 0
+        full = np.ctypeslib.as_array((ctypes.c_double * (11 * 9)).from_address(ptr.value)).reshape(11, 9, order="F")
+        assert np.array_equal(full[3:8, 3:6], want)
     # wrong grid and missing file -> -3
+    with netcdf_file(str(tmp_path / "w.nc"), "w", version=1) as nc:
+        nc.createDimension("x", 4)
+        nc.createDimension("y", 3)
+    pwn = str(tmp_path / "w.nc").encode()
+    assert L.rki_load_restart(h, pwn, len(pwn)) == -3
     write_restart(tmp_path / "w.rkrs", 4, 3, 3, 0.0, [np.zeros((4, 3))] * 4)
     pw = str(tmp_path / "w.rkrs").encode()
     assert L.rki_load_restart(h, pw, len(pw)) == -3
