@@ -66,6 +66,8 @@ def parse():
                     help="members of the batched-adjoint run (0 disables)")
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-fd", type=int, default=8, help="finite-difference directions per member")
+    ap.add_argument("--feature-steps", type=int, default=50,
+                    help="steps per optional-source-term variant of the 4096^2 step (0 disables)")
     return ap.parse_args()
 
 
@@ -154,6 +156,8 @@ def main():
             "frac_of_copy": achieved / copy_gbs if copy_gbs else None,
         },
     }
+    if args.feature_steps > 0:
+        out["source_terms"] = run_source_terms(scn, args.feature_steps)
     if args.ensemble_members > 0:
         out["ensemble"] = run_ensemble(args.ensemble_members, args.ensemble_steps)
     if args.adjoint_n > 0:
@@ -176,6 +180,55 @@ def measured_traffic(stage: int):
         return None
     m = re.search(r"k_stage<%d>=(\d+)" % stage, p.read_text())
     return int(m.group(1)) if m else None
+
+
+def run_source_terms(scn, steps: int, warmup: int = 5):
+    """Stage-kernel time of the same 4096^2 step with the optional source
+    terms on: surface forcing (wind, pressure, rain), a 64-cell sponge band
+    on the east edge, a Chapman east edge, and all three.  Each set runs its
+    own kernel instantiation; the plain kernels do not change."""
+    import numpy as np
+    from rakali_amd import Grid
+    x = (np.arange(scn.nx) + 0.5)[:, None] * scn.dx
+    y = (np.arange(scn.ny) + 0.5)[None, :] * scn.dy
+    forcing = {
+        "wind_x": np.asfortranarray(np.full((scn.nx, scn.ny), 0.1)),
+        "wind_y": np.asfortranarray(np.full((scn.nx, scn.ny), -0.05)),
+        "atm_pressure": np.asfortranarray(101325.0 - 0.01 * x - 0.005 * y),
+        "precipitation": np.asfortranarray(np.full((scn.nx, scn.ny), 1e-6)),
+    }
+    variants = {
+        "plain": dict(),
+        "forcing": dict(forcing=True),
+        "sponge": dict(bc=("tidal", "sponge", "wall", "wall"), sponge=(64, 1e-3, 30.0)),
+        "chapman": dict(bc=("tidal", "chapman", "wall", "wall")),
+        "all": dict(bc=("tidal", "chapman", "sponge", "wall"), sponge=(64, 1e-3, 30.0), forcing=True),
+    }
+    res = {}
+    for name, v in variants.items():
+        kw = dict(scn.grid_kw)
+        if "bc" in v:
+            kw["bc"] = v["bc"]
+        g = Grid(scn.nx, scn.ny, scn.dx, scn.dy, **kw)
+        for k in ("b", "h", "hu", "hv"):
+            g.set_field(k, getattr(scn, k))
+        if "sponge" in v:
+            g.sponge = v["sponge"]
+        if v.get("forcing"):
+            g.forcing = True
+            for k, a in forcing.items():
+                g.set_field(k, a)
+        g.step(warmup)
+        g.profile(True)
+        g.step(steps)
+        ms1, ms2, n = g.profile_read()
+        g.close()
+        res[name] = {"stage1_ms": ms1 / n, "stage2_ms": ms2 / n, "step_ms": (ms1 + ms2) / n}
+    base = res["plain"]["step_ms"]
+    for name in res:
+        res[name]["vs_plain"] = res[name]["step_ms"] / base
+    return {"workload": f"the 4096x4096 step above, {steps} steps per variant (stage kernels, HIP events)",
+            "variants": res}
 
 
 def run_ensemble(n_members: int, steps: int, warmup: int = 2):
