@@ -204,8 +204,14 @@ struct Reader {
             if (d.first == n) return d.second;
         return 0;
     }
+    // Index of the named dimension, or -1.
+    int dim_id(const std::string& n) const {
+        for (size_t k = 0; k < dims.size(); ++k)
+            if (dims[k].first == n) return (int)k;
+        return -1;
+    }
     bool read_doubles(const Var& v, size_t count, double* out) const {
-        if (v.type != NC_DOUBLE || v.begin + count * 8 > buf.size()) return false;
+        if (v.type != NC_DOUBLE || v.begin > buf.size() || count > (buf.size() - v.begin) / 8) return false;
         const uint8_t* p = buf.data() + v.begin;
         for (size_t k = 0; k < count; ++k) {
             uint64_t u = 0;

@@ -1500,7 +1500,8 @@ int rki_load_restart(void* handle, const char* filename, int len) {
             double t = 0.0;
             for (int k = 0; k < 4; ++k) {
                 const rk::nc::Var* v = r.find(names[k]);
-                if (!v || v->dims.size() != 2 || !r.read_doubles(*v, n, buf.data() + k * n)) {
+                if (!v || v->dims.size() != 2 || (int)v->dims[0] != r.dim_id("y") ||
+                    (int)v->dims[1] != r.dim_id("x") || !r.read_doubles(*v, n, buf.data() + k * n)) {
                     set_error("restart file '%s': variable %s missing or not FP64 (y, x)", path.c_str(), names[k]);
                     return -3;
                 }

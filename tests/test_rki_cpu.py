@@ -235,6 +235,15 @@ def test_load_restart_phase1(L, tmp_path):
         assert getattr(L, getter)(h, ctypes.byref(ptr), ctypes.byref(nxt), ctypes.byref(nyt)) == 0
         full = np.ctypeslib.as_array((ctypes.c_double * (11 * 9)).from_address(ptr.value)).reshape(11, 9, order="F")
         assert np.array_equal(full[3:8, 3:6], want)
+    # fields stored as (x, y) instead of restart_write's (y, x) -> -3
+    with netcdf_file(str(tmp_path / "t.nc"), "w", version=2) as nc:
+        nc.createDimension("x", 5)
+        nc.createDimension("y", 3)
+        for name in ("h", "hu", "hv", "b"):
+            nc.createVariable(name, "d", ("x", "y"))[:, :] = 1.0
+        nc.createVariable("time", "d", ())[...] = 0.0
+    pt = str(tmp_path / "t.nc").encode()
+    assert L.rki_load_restart(h, pt, len(pt)) == -3
     # wrong grid and missing file -> -3
     with netcdf_file(str(tmp_path / "w.nc"), "w", version=1) as nc:
         nc.createDimension("x", 4)
