@@ -204,8 +204,10 @@ int rkg_get_mass_budget(rkg_grid *g, double *mass_out);
  * stage adds the wind stress and the pressure gradient to hu, hv of wet
  * cells and the precipitation to h of every cell (rki_kernel_fused.F90:
  * 273-289).  The first call allocates the four RKG_FIELD_WIND_X..PRECIP
- * planes (32 B per cell); they stay until rkg_destroy.  The adjoint does
- * not support forcing. */
+ * planes and two for the pressure gradient (48 B per cell); they stay until
+ * rkg_destroy.  The gradient is taken from the pressure plane before the
+ * next step after an upload of it (every step once its device pointer has
+ * been requested).  The adjoint does not support forcing. */
 /* bc%time and bc%dt of the last step's last boundary call (stage 2 at
  * t + dt for SSP-RK2, t + c_last dt for a generic scheme). */
 int rkg_get_last_bc(rkg_grid *g, double *time, double *dt);

@@ -208,6 +208,15 @@ int launch_steps_now(rkg_grid* g, int n, int mode, double tlim) {
         int rc = refresh_speed(g);
         if (rc) return rc;
     }
+    if (g->forcing && (g->frc_grad_stale || g->frc_pres_out)) {
+        const long long cells = (long long)g->cfg.nx * g->cfg.ny;
+        const int blocks = (int)std::min<long long>((cells + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_pressure_gradient, dim3(blocks), dim3(256), 0, g->stream, g->frc + 2 * g->plane,
+                           g->frc + 4 * g->plane, g->frc + 5 * g->plane, g->cfg.nx, g->cfg.ny, (long long)g->pitch,
+                           g->cfg.dx, g->cfg.dy);
+        HIP_TRY(hipGetLastError());
+        g->frc_grad_stale = false;
+    }
     if (g->scheme != RKG_TIME_RK2) return launch_generic(g, n, mode, tlim);
     GridDesc d = make_desc(g);
     const int ntiles = d.nstrips * d.nsegs;
@@ -382,6 +391,7 @@ static int copy_field(rkg_grid* g, int field, double* host_or_dev, int ld, bool 
         HIP_TRY(hipMemcpy2DAsync(f, (size_t)g->pitch * sizeof(double), host_or_dev, (size_t)ld * sizeof(double), w,
                                  (size_t)g->cfg.ny, kind, g->stream));
         g->speed_stale = true;
+        if (field == RKG_FIELD_PRESSURE) g->frc_grad_stale = true;
     } else {
         HIP_TRY(hipMemcpy2DAsync(host_or_dev, (size_t)ld * sizeof(double), f, (size_t)g->pitch * sizeof(double), w,
                                  (size_t)g->cfg.ny, kind, g->stream));
@@ -407,6 +417,9 @@ int rkg_field_device_ptr(rkg_grid* g, int field, double** ptr, long long* pitch)
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     double* f = field_ptr(g, field);
     if (!f || !ptr || !pitch) return RKG_ERR_RANGE;
+    // the caller may write the pressure at any time from now on: the
+    // gradient planes are then refreshed before every launch
+    if (field == RKG_FIELD_PRESSURE) g->frc_pres_out = true;
     *ptr = f;
     *pitch = g->pitch;
     return RKG_OK;
@@ -547,12 +560,13 @@ __global__ void k_fill(double* p, long long n, double v) {
 int rkg_set_forcing(rkg_grid* g, int active) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     if (active && !g->frc) {
-        HIP_TRY(hipMalloc((void**)&g->frc, 4 * g->plane * sizeof(double)));
-        hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, g->stream, g->frc, (long long)(4 * g->plane), 0.0);
+        HIP_TRY(hipMalloc((void**)&g->frc, 6 * g->plane * sizeof(double)));
+        hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, g->stream, g->frc, (long long)(6 * g->plane), 0.0);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(g->stream));
     }
     g->forcing = active != 0;
+    g->frc_grad_stale = true;
     return RKG_OK;
 }
 

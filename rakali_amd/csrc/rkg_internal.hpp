@@ -37,8 +37,11 @@ struct rkg_grid {
                          // change of h (2 x mpart_cap each), or null
     long long mpart_cap; // tiles per stage that mpart holds
     int scheme;          // RKG_TIME_*
-    double* frc;         // 4 planes: wind stress x, y, atmospheric pressure, precipitation, or null
+    double* frc;         // 6 planes: wind stress x, y, atmospheric pressure, precipitation, and
+                         // the pressure gradient x, y (from the pressure plane), or null
     bool forcing;        // forcing%active: the stage kernels apply the planes in frc
+    bool frc_grad_stale; // the pressure plane may have changed since the gradient was taken
+    bool frc_pres_out;   // a device pointer to the pressure plane was handed out
     int sponge_width;    // sponge relaxation next to RKG_BC_SPONGE edges (bc_params_t)
     double sponge_strength, sponge_h_ref;
     rk::CtrlSlot* ctrl;      // device [2]
@@ -177,8 +180,9 @@ inline GridDesc make_desc(rkg_grid* g) {
     if (g->forcing) {
         d.fwx = g->frc;
         d.fwy = g->frc + g->plane;
-        d.fpres = g->frc + 2 * g->plane;
         d.frain = g->frc + 3 * g->plane;
+        d.fdpx = g->frc + 4 * g->plane;
+        d.fdpy = g->frc + 5 * g->plane;
     }
     d.ssp_coeff = g->scheme == RKG_TIME_RK43 ? 2.0 : 1.0;  // rki_timestepper.F90:60-100
     return d;

@@ -306,6 +306,25 @@ __global__ __launch_bounds__(256) void k_mass_partial(GridDesc d, double* partia
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// The central pressure gradient of the surface forcing
+// (rki_kernel_fused.F90:277-278), dp/dx = (p(i+1) - p(i-1)) / (2 dx), with
+// pressure 0 outside the grid (the ghost cells, which no setter writes);
+// computed once per pressure field, in the same operations as per stage.
+__global__ __launch_bounds__(256) void k_pressure_gradient(const double* p, double* dpx, double* dpy, int nx,
+                                                           int ny, long long pitch, double dx, double dy) {
+    const long long n = (long long)nx * ny;
+    for (long long q = blockIdx.x * 256ll + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+        const int c = (int)(q % nx), j = (int)(q / nx);
+        const long long k = (long long)j * pitch + c;
+        const double pe = c + 1 < nx ? p[k + 1] : 0.0;
+        const double pw = c > 0 ? p[k - 1] : 0.0;
+        const double pn = j + 1 < ny ? p[k + pitch] : 0.0;
+        const double ps = j > 0 ? p[k - pitch] : 0.0;
+        dpx[k] = (pe - pw) / (2.0 * dx);
+        dpy[k] = (pn - ps) / (2.0 * dy);
+    }
+}
+
 // Per-block sums of the interior of one plane (rows strided over blocks),
 // added up by k_mass_final: a fixed order, so the result is deterministic.
 __global__ __launch_bounds__(256) void k_plane_sum_partial(const double* f, long long pitch, int nx, int ny,

@@ -51,7 +51,8 @@ struct GridDesc {
     double* mpart;                // per-tile sums of flux_h, [stage 1 tiles | stage 2 tiles], or null
     double* hist;                 // recording stage 1: (t, dt, dt recomputed) of the step, or null
     const double* fwx; const double* fwy;   // surface forcing planes (FEAT_FORCING kernels): wind
-    const double* fpres; const double* frain; // stress, atmospheric pressure, precipitation
+    const double* fdpx; const double* fdpy; // stress, the central pressure gradient (from the
+    const double* frain;                    // pressure field, see k_pressure_gradient), precipitation
     double* spart;                // per-tile sums of the sponge's h change, [stage 1 | stage 2], or null
     int sponge_width;             // sponge relaxation band (FEAT_SPONGE kernels), bc_params_t
     double sponge_strength, sponge_h_ref;
@@ -500,15 +501,11 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                 }
                 double h = rc.h, hu = rc.hu, hv = rc.hv;
                 Frc fr{0.0, 0.0, 0.0, 0.0, 0.0};
-                if (FRC) {   // pressure off the grid is the ghost value, which no setter writes: 0
+                if (FRC) {
                     fr.wx = d.fwx[k];
                     fr.wy = d.fwy[k];
-                    const double pe = c + 1 < d.nx ? d.fpres[k + 1] : 0.0;
-                    const double pw = c > 0 ? d.fpres[k - 1] : 0.0;
-                    const double pn = j + 1 < d.ny ? d.fpres[k + pitch] : 0.0;
-                    const double ps = j > 0 ? d.fpres[k - pitch] : 0.0;
-                    fr.dpdx = (pe - pw) / (2.0 * d.dx);
-                    fr.dpdy = (pn - ps) / (2.0 * d.dy);
+                    fr.dpdx = d.fdpx[k];
+                    fr.dpdy = d.fdpy[k];
                     fr.rain = d.frain[k];
                 }
                 if (NF)
