@@ -349,7 +349,10 @@ int rkg_adjoint_forward(rkg_adjoint *a, double *loss);
 int rkg_adjoint_gauge_eta(rkg_adjoint *a, double *eta);
 /* gradients of loss_scale * L; tile_rows of the reverse y-face kernel (0:
  * chosen from the grid size so that about 1900 tiles run at once, 4 to 32
- * rows) */
+ * rows).  With two or more segments, the first call also allocates a
+ * second segment workspace ((12 K + 3) planes) and a stream of its own when
+ * that takes at most half of the free device memory: segment s - 1 is then
+ * replayed while segment s is reversed (the same gradients, bit for bit). */
 int rkg_adjoint_backward(rkg_adjoint *a, double loss_scale, int tile_rows);
 /* adds the bed part of a still-water initial state h = max(eta0 - b, 0) */
 int rkg_adjoint_fold_still_water(rkg_adjoint *a, double eta0);

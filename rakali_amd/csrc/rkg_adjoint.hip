@@ -452,34 +452,58 @@ __global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
     }
 }
 
-// This thread's share of d L / d eta_tidal of one stage from its tidal
-// ghost cells (edge_lg; stage input `in`, 3 planes): the eta_ext terms of
-// bc_flather_edge (rki_boundary.F90:331-343), h_g = max(eta_ext - b_g, 0)
-// and the radiation term.
-__device__ double eta_part(const GridDesc& d, double eta, const double* in, const double* edge_lg, int len) {
-    double acc = 0.0;
+// d L / d eta_tidal from one tidal ghost cell (adjoints lg0 of h and lg1
+// of the normal momentum; in0 its source cell of the stage input): the
+// eta_ext terms of bc_flather_edge (rki_boundary.F90:331-343),
+// h_g = max(eta_ext - b_g, 0) and the radiation term.
+__device__ double eta_rev_one(const GridDesc& d, double eta, const Cell& in0, double lg0, double lg1, double sgn,
+                              bool xe) {
+    adj::C4 lg{lg0, 0.0, 0.0, 0.0};
+    if (xe) lg.hu = lg1;
+    else lg.hv = lg1;
+    adj::C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
+    double le = 0.0;
+    adj::ghost_rev(BC_TIDAL, d.g, eta, d.inflow_velocity, d.clamped_elevation, in0, in0, sgn, xe, lg, lm, li, &le);
+    return le;
+}
+
+// This thread's share for both stages at once (stage 1: input d.ah at
+// eta1, ghost adjoints lg1; stage 2: input in2 at eta2, lg2).  Items
+// i = threadIdx.x + q blockDim.x go two at a time, their loads issued
+// together; each stage's sum keeps the item order.
+__device__ void eta_parts(const GridDesc& d, double eta1, double eta2, const double* in2, const double* lg1,
+                          const double* lg2, int len, double* acc1, double* acc2) {
+    double s1 = 0.0, s2 = 0.0;
     for (int e = 0; e < 4; ++e) {
         if (d.bc[e] != BC_TIDAL) continue;
         const bool xe = e < 2;
         const int n = xe ? d.ny : d.nx;
         const double sgn = (e == 0 || e == 2) ? -1.0 : 1.0;
-        for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) {
-            const int kk = i / n, p = i % n;
-            const int ci = xe ? (e == 0 ? 0 : d.nx - 1) : p;
-            const int cj = xe ? p : (e == 2 ? 0 : d.ny - 1);
-            const Cell in0 = load_direct(in, in + d.plane, in + 2 * d.plane, d.bed, d.pitch, ci, cj);
-            const long long o = edge_slot(e, kk, p, len);
-            adj::C4 lg{edge_lg[o], 0.0, 0.0, 0.0};
-            if (xe) lg.hu = edge_lg[o + 1];
-            else lg.hv = edge_lg[o + 1];
-            adj::C4 lm{0, 0, 0, 0}, li{0, 0, 0, 0};
-            double le = 0.0;
-            adj::ghost_rev(BC_TIDAL, d.g, eta, d.inflow_velocity, d.clamped_elevation, in0, in0, sgn, xe, lg, lm, li,
-                           &le);
-            acc = acc + le;
+        for (int i0 = threadIdx.x; i0 < 2 * n; i0 += 2 * blockDim.x) {
+            Cell c1[2], c2[2];
+            double g[2][4];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int i = u == 0 ? i0 : min(i0 + (int)blockDim.x, 2 * n - 1);
+                const int kk = i / n, p = i % n;
+                const int ci = xe ? (e == 0 ? 0 : d.nx - 1) : p;
+                const int cj = xe ? p : (e == 2 ? 0 : d.ny - 1);
+                c1[u] = load_direct(d.ah, d.ahu, d.ahv, d.bed, d.pitch, ci, cj);
+                c2[u] = load_direct(in2, in2 + d.plane, in2 + 2 * d.plane, d.bed, d.pitch, ci, cj);
+                const long long o = edge_slot(e, kk, p, len);
+                g[u][0] = lg1[o]; g[u][1] = lg1[o + 1];
+                g[u][2] = lg2[o]; g[u][3] = lg2[o + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (u == 1 && i0 + (int)blockDim.x >= 2 * n) break;
+                s1 = s1 + eta_rev_one(d, eta1, c1[u], g[u][0], g[u][1], sgn, xe);
+                s2 = s2 + eta_rev_one(d, eta2, c2[u], g[u][2], g[u][3], sgn, xe);
+            }
         }
     }
-    return acc;
+    *acc1 = s1;
+    *acc2 = s2;
 }
 
 // Gauges of a session: cells (eta = h + b of the cell) or bilinear
@@ -568,22 +592,51 @@ __global__ __launch_bounds__(1024) void k_dt_rev(GridDesc d, const double* hist,
     __shared__ double red[3][1024];
     __shared__ double sv[1024];
     __shared__ long long si[1024];
+    __shared__ double tide[4];   // eta(t), eta(t + dt), eta'(t), eta'(t + dt)
     const double t = hist[0], dt = hist[1];
+    // The kernel is bound by dependent memory round trips: each loop below
+    // issues four elements' loads before it waits, and adds or compares them
+    // in the element order of a plain strided loop.
     double a = 0.0;
-    for (int i = threadIdx.x; i < 2 * nbody; i += 1024) a = a + part_dt[i];
-    red[0][threadIdx.x] = a;
-    // d L / d eta_tidal of stage 1 (input A = d.ah, at t) and stage 2 (input B = in2, at t + dt)
-    red[1][threadIdx.x] = eta_part(d, tidal_eta(d, t), d.ah, edge_lg, edge_len);
-    red[2][threadIdx.x] = eta_part(d, tidal_eta(d, t + dt), in2, edge_lg + edge_slot(4, 0, 0, edge_len), edge_len);
+    for (int i0 = threadIdx.x; i0 < 2 * nbody; i0 += 4 * 1024) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = part_dt[min(i0 + u * 1024, 2 * nbody - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * 1024 < 2 * nbody) a = a + v[u];
+    }
     double best = 0.0;
     long long bi = -1;
-    for (int i = threadIdx.x; i < nargmax; i += 1024) {
-        const double v = pval[i];
-        const long long x = pidx[i];
-        if (v > best || (v == best && x >= 0 && (bi < 0 || x < bi))) { best = v; bi = x; }
+    for (int i0 = threadIdx.x; i0 < nargmax; i0 += 4 * 1024) {
+        double v[4];
+        long long x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = min(i0 + u * 1024, nargmax - 1);
+            v[u] = pval[i];
+            x[u] = pidx[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * 1024 < nargmax && (v[u] > best || (v[u] == best && x[u] >= 0 && (bi < 0 || x[u] < bi)))) {
+                best = v[u];
+                bi = x[u];
+            }
     }
+    red[0][threadIdx.x] = a;
     sv[threadIdx.x] = best;
     si[threadIdx.x] = bi;
+    const int w = threadIdx.x / WAVE;
+    if (w < 4) {
+        const double tw = (w & 1) ? t + dt : t;
+        const double v = w < 2 ? tidal_eta(d, tw) : adj::tidal_deta(d, tw);
+        if (threadIdx.x % WAVE == 0) tide[w] = v;
+    }
+    __syncthreads();
+    // d L / d eta_tidal of stage 1 (input A = d.ah, at t) and stage 2 (input B = in2, at t + dt)
+    eta_parts(d, tide[0], tide[1], in2, edge_lg, edge_lg + edge_slot(4, 0, 0, edge_len), edge_len, &red[1][threadIdx.x],
+              &red[2][threadIdx.x]);
     __syncthreads();
     for (int s = 512; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
@@ -599,9 +652,9 @@ __global__ __launch_bounds__(1024) void k_dt_rev(GridDesc d, const double* hist,
     }
     if (threadIdx.x != 0) return;
     const bool recomputed = hist[2] != 0.0;
-    const double lam_t1 = scal[0] + red[2][0] * adj::tidal_deta(d, t + dt);   // stage 2 ran at t + dt
+    const double lam_t1 = scal[0] + red[2][0] * tide[3];   // stage 2 ran at t + dt
     const double gdt = red[0][0] + lam_t1 + scal[1];
-    double lam_t = lam_t1 + red[1][0] * adj::tidal_deta(d, t);
+    double lam_t = lam_t1 + red[1][0] * tide[2];
     double carry = 0.0;
     const double dt_max = d.t_end - t;
     if (d.dt_fixed > 0.0) {
@@ -615,11 +668,12 @@ __global__ __launch_bounds__(1024) void k_dt_rev(GridDesc d, const double* hist,
         } else if (ms > 0.0 && si[0] >= 0) {                 // dt = cfl ssp / max speed
             const long long idx = si[0];
             const long long k = (idx / d.nx) * d.pitch + idx % d.nx;
+            const double l0 = lam[k], l1 = lam[d.plane + k], l2 = lam[2 * d.plane + k];
             const double lms = gdt * (-(d.cfl * d.ssp_coeff) / (ms * ms));
             const adj::S3 l = adj::cfl_speed_rev(d.g, d.dx, d.dy, d.ah[k], d.ahu[k], d.ahv[k], lms);
-            lam[k] += l.h;
-            lam[d.plane + k] += l.hu;
-            lam[2 * d.plane + k] += l.hv;
+            lam[k] = l0 + l.h;
+            lam[d.plane + k] = l1 + l.hu;
+            lam[2 * d.plane + k] = l2 + l.hv;
         }
     } else {
         carry = gdt;                                         // dt reused from the step before
@@ -673,6 +727,16 @@ struct rkg_adjoint {
     bool have_forward;
     float fwd_ms, bwd_ms;
     hipEvent_t ev[2];
+    // Overlapped replay (rkg_adjoint_backward): a second workspace and
+    // history, so that segment s - 1 is replayed on rstream while segment s
+    // is reversed on the grid's stream.  ws / hist and ws_alt / hist_alt
+    // swap as the sweep moves between the sets; cur_set is the set in ws.
+    double* ws_alt;
+    double* hist_alt;
+    int cur_set;
+    bool overlap_off;           // the second set did not fit, or a single segment
+    hipStream_t rstream;
+    hipEvent_t seg_ev[4];       // replay into set 0 / 1 done, reverse of set 0 / 1 done
 };
 
 namespace {
@@ -701,13 +765,13 @@ GridDesc stage_desc(rkg_adjoint* a, double* in, double* in2, double* out2) {
 }
 
 template <int STAGE>
-void launch_rec(rkg_grid* g, const GridDesc& d, int p) {
+void launch_rec(rkg_grid* g, const GridDesc& d, int p, hipStream_t st) {
     const int ntiles = d.nstrips * d.nsegs;
     const double tlim = g->cfg.t_end;
     if (g->cfg.use_hllc)
-        hipLaunchKernelGGL((k_stage<STAGE, true, true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, p, MODE_STEP, tlim);
+        hipLaunchKernelGGL((k_stage<STAGE, true, true, true>), dim3(ntiles), dim3(WAVE), 0, st, d, p, MODE_STEP, tlim);
     else
-        hipLaunchKernelGGL((k_stage<STAGE, false, true, true>), dim3(ntiles), dim3(WAVE), 0, g->stream, d, p, MODE_STEP, tlim);
+        hipLaunchKernelGGL((k_stage<STAGE, false, true, true>), dim3(ntiles), dim3(WAVE), 0, st, d, p, MODE_STEP, tlim);
 }
 
 // The stage's face reverse: k_adj_y then k_adj_x (both add into lam_in and
@@ -901,11 +965,15 @@ void rkg_adjoint_destroy(rkg_adjoint** pa) {
     rkg_adjoint* a = *pa;
     if (a->magic != ADJ_MAGIC) { *pa = nullptr; return; }
     if (a->g && a->g->stream) (void)hipStreamSynchronize(a->g->stream);
+    if (a->rstream) (void)hipStreamSynchronize(a->rstream);
     for (void* p : {(void*)a->gi, (void*)a->gj, (void*)a->gw, (void*)a->cell_start, (void*)a->cell_gauges,
                     (void*)a->cell_w, (void*)a->cell_pos, (void*)a->tgt, (void*)a->eta, (void*)a->ckpt, (void*)a->ckpt_ctrl,
                     (void*)a->ws, (void*)a->lam, (void*)a->hist, (void*)a->loss_dev, (void*)a->part,
-                    (void*)a->part_idx, (void*)a->scal, (void*)a->edge_lg})
+                    (void*)a->part_idx, (void*)a->scal, (void*)a->edge_lg, (void*)a->ws_alt, (void*)a->hist_alt})
         if (p) (void)hipFree(p);
+    if (a->rstream) (void)hipStreamDestroy(a->rstream);
+    for (hipEvent_t ev : a->seg_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (a->loss_host) (void)hipHostFree(a->loss_host);
     if (a->ev[0]) (void)hipEventDestroy(a->ev[0]);
     if (a->ev[1]) (void)hipEventDestroy(a->ev[1]);
@@ -1047,30 +1115,64 @@ int bwd_begin(rkg_adjoint* a, double loss_scale, int tile_rows, BwdPlan& pl) {
 
 int seg_steps(const rkg_adjoint* a, int s) { return std::min(a->nsteps, (s + 1) * a->K) - s * a->K; }
 
-// Start of the replay of segment s: its checkpoint into A_0 and the clock.
-int bwd_replay_begin(rkg_adjoint* a, int s, BwdPlan& pl) {
+// Start of the replay of segment s on stream st: its checkpoint into A_0
+// and the clock.
+int bwd_replay_begin(rkg_adjoint* a, int s, BwdPlan& pl, hipStream_t st) {
     rkg_grid* g = a->g;
     const size_t Pl = g->plane;
     HIP_TRY(hipMemcpyAsync(wsA(a, 0), a->ckpt + (size_t)s * 3 * Pl, 3 * Pl * sizeof(double), hipMemcpyDeviceToDevice,
-                           g->stream));
-    HIP_TRY(hipMemcpyAsync(g->ctrl, a->ckpt_ctrl + 2 * s, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice, g->stream));
+                           st));
+    HIP_TRY(hipMemcpyAsync(g->ctrl, a->ckpt_ctrl + 2 * s, 2 * sizeof(CtrlSlot), hipMemcpyDeviceToDevice, st));
     pl.p = a->ckpt_parity[s];
     return RKG_OK;
 }
 
 // Step k of the segment's replay, recording U^n, U*, both divergences and
 // the clock.
-void bwd_replay_step(rkg_adjoint* a, int k, BwdPlan& pl) {
+void bwd_replay_step(rkg_adjoint* a, int k, BwdPlan& pl, hipStream_t st) {
     rkg_grid* g = a->g;
     GridDesc d1 = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
     d1.rec = wsD1(a, k);
     d1.hist = a->hist + HIST * k;
-    launch_rec<1>(g, d1, pl.p);
+    launch_rec<1>(g, d1, pl.p, st);
     GridDesc d2 = d1;
     d2.rec = wsD2(a, k);
     d2.hist = nullptr;
-    launch_rec<2>(g, d2, pl.p);
+    launch_rec<2>(g, d2, pl.p, st);
     pl.p ^= 1;
+}
+
+// Puts workspace set `set` (0 or 1) in ws / hist.
+void use_set(rkg_adjoint* a, int set) {
+    if (set == a->cur_set) return;
+    std::swap(a->ws, a->ws_alt);
+    std::swap(a->hist, a->hist_alt);
+    a->cur_set = set;
+}
+
+size_t ws_bytes(const rkg_adjoint* a) { return (size_t)(12 * a->K + 3) * a->g->plane * sizeof(double); }
+
+// The second workspace, stream and events of the overlapped replay,
+// allocated at the first backward sweep that can use them: a session of
+// two or more segments whose second workspace takes at most half of the
+// device's free memory.  Otherwise the sweep replays on the grid's stream.
+int ensure_overlap(rkg_adjoint* a) {
+    if (a->rstream || a->overlap_off) return RKG_OK;
+    if (a->nseg < 2) { a->overlap_off = true; return RKG_OK; }
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t hist_b = (size_t)a->K * HIST * sizeof(double);
+    if (ws_bytes(a) + hist_b > free_b / 2) { a->overlap_off = true; return RKG_OK; }
+    hipError_t e = hipMalloc((void**)&a->ws_alt, ws_bytes(a));
+    if (e == hipSuccess) e = hipMalloc((void**)&a->hist_alt, hist_b);
+    if (e == hipSuccess) e = hipMemset(a->ws_alt, 0, ws_bytes(a));
+    for (int q = 0; q < 4 && e == hipSuccess; ++q) e = hipEventCreateWithFlags(&a->seg_ev[q], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->rstream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_error("adjoint workspace allocation failed: %s", hipGetErrorString(e));
+        return RKG_ERR_HIP;
+    }
+    return RKG_OK;
 }
 
 // Reverse of step k of segment s (global step n0 + k).
@@ -1124,6 +1226,41 @@ void bwd_reverse_step(rkg_adjoint* a, int s, int k, const BwdPlan& pl) {
     hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(1024), 0, g->stream, st.d, st.hist, part_dt, nb,
                        (const double*)wsB(a, k), (const double*)a->edge_lg, a->edge_len, pval, a->part_idx,
                        pl.nargmax, lamA(a), a->scal);
+}
+
+// The sweep with the replay of segment s - 1 (into workspace set (s-1)&1,
+// on rstream) overlapping the reverse of segment s (set s&1, on the grid's
+// stream).  Events order each set's replay before its reverse and a set's
+// next replay after the reverse that last read it; the kernels and their
+// inputs are those of the serial sweep, so the gradients are the same bits.
+int bwd_overlapped(rkg_adjoint* a, BwdPlan& pl) {
+    hipStream_t main = a->g->stream, rs = a->rstream;
+    // the replays start after the work already queued on the grid's stream
+    HIP_TRY(hipEventRecord(a->seg_ev[2], main));
+    HIP_TRY(hipStreamWaitEvent(rs, a->seg_ev[2], 0));
+    auto replay = [&](int s) -> int {
+        use_set(a, s & 1);
+        int rc = bwd_replay_begin(a, s, pl, rs);
+        if (rc) return rc;
+        for (int k = 0; k < seg_steps(a, s); ++k) bwd_replay_step(a, k, pl, rs);
+        HIP_TRY(hipEventRecord(a->seg_ev[s & 1], rs));
+        return RKG_OK;
+    };
+    int rc = replay(a->nseg - 1);
+    for (int s = a->nseg - 1; s >= 0 && !rc; --s) {
+        if (s > 0) {
+            // set (s-1)&1 was last read by the reverse of segment s + 1
+            if (s + 1 < a->nseg) HIP_TRY(hipStreamWaitEvent(rs, a->seg_ev[2 + ((s + 1) & 1)], 0));
+            rc = replay(s - 1);
+            if (rc) break;
+        }
+        HIP_TRY(hipStreamWaitEvent(main, a->seg_ev[s & 1], 0));
+        use_set(a, s & 1);
+        for (int k = seg_steps(a, s) - 1; k >= 0; --k) bwd_reverse_step(a, s, k, pl);
+        HIP_TRY(hipEventRecord(a->seg_ev[2 + (s & 1)], main));
+    }
+    use_set(a, 0);
+    return rc;
 }
 
 int bwd_end(rkg_adjoint* a) {
@@ -1181,15 +1318,23 @@ int rkg_adjoint_forward(rkg_adjoint* a, double* loss) {
 // is left as the forward run left it.
 int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
     if (!valid(a)) return RKG_ERR_BAD_HANDLE;
-    BwdPlan pl;
-    int rc = bwd_begin(a, loss_scale, tile_rows, pl);
+    if (!a->have_forward) { set_error("rkg_adjoint_backward before rkg_adjoint_forward"); return RKG_ERR_NOT_READY; }
+    int rc = ensure_overlap(a);   // before the sweep's start event: the first call allocates
     if (rc) return rc;
-    for (int s = a->nseg - 1; s >= 0; --s) {
-        rc = bwd_replay_begin(a, s, pl);
-        if (rc) return rc;
-        for (int k = 0; k < seg_steps(a, s); ++k) bwd_replay_step(a, k, pl);
-        for (int k = seg_steps(a, s) - 1; k >= 0; --k) bwd_reverse_step(a, s, k, pl);
+    BwdPlan pl;
+    rc = bwd_begin(a, loss_scale, tile_rows, pl);
+    if (rc) return rc;
+    if (a->rstream) {
+        rc = bwd_overlapped(a, pl);
+    } else {
+        for (int s = a->nseg - 1; s >= 0 && !rc; --s) {
+            rc = bwd_replay_begin(a, s, pl, a->g->stream);
+            if (rc) break;
+            for (int k = 0; k < seg_steps(a, s); ++k) bwd_replay_step(a, k, pl, a->g->stream);
+            for (int k = seg_steps(a, s) - 1; k >= 0; --k) bwd_reverse_step(a, s, k, pl);
+        }
     }
+    if (rc) return rc;
     rc = bwd_end(a);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(a->g->stream));
@@ -1220,11 +1365,11 @@ int rkg_adjoint_backward_many(int n, rkg_adjoint* const* as, double loss_scale) 
     std::vector<BwdPlan> pl(n);
     for (int m = 0; m < n && !rc; ++m) rc = bwd_begin(as[m], loss_scale, 0, pl[m]);
     for (int s = as[0]->nseg - 1; s >= 0 && !rc; --s) {
-        for (int m = 0; m < n && !rc; ++m) rc = bwd_replay_begin(as[m], s, pl[m]);
+        for (int m = 0; m < n && !rc; ++m) rc = bwd_replay_begin(as[m], s, pl[m], as[m]->g->stream);
         if (rc) break;
         const int Ks = seg_steps(as[0], s);
         for (int k = 0; k < Ks; ++k)
-            for (int m = 0; m < n; ++m) bwd_replay_step(as[m], k, pl[m]);
+            for (int m = 0; m < n; ++m) bwd_replay_step(as[m], k, pl[m], as[m]->g->stream);
         for (int k = Ks - 1; k >= 0; --k)
             for (int m = 0; m < n; ++m) bwd_reverse_step(as[m], s, k, pl[m]);
     }

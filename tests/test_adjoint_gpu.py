@@ -163,6 +163,10 @@ def test_checkpoint_interval_does_not_change_gradients():
         loss = sess.forward()
         sess.backward()
         outs.append((loss, sess.gradient("b"), sess.gradient("n")))
+        # K < steps replays segment s - 1 on a second stream and workspace
+        # while segment s is reversed; a second sweep reuses them
+        sess.backward()
+        assert np.array_equal(sess.gradient("b"), outs[-1][1]) and np.array_equal(sess.gradient("n"), outs[-1][2])
         sess.close()
         g.close()
     for loss, gb, gn in outs[1:]:
