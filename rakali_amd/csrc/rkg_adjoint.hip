@@ -1155,24 +1155,32 @@ size_t ws_bytes(const rkg_adjoint* a) { return (size_t)(12 * a->K + 3) * a->g->p
 // The second workspace, stream and events of the overlapped replay,
 // allocated at the first backward sweep that can use them: a session of
 // two or more segments whose second workspace takes at most half of the
-// device's free memory.  Otherwise the sweep replays on the grid's stream.
-int ensure_overlap(rkg_adjoint* a) {
-    if (a->rstream || a->overlap_off) return RKG_OK;
-    if (a->nseg < 2) { a->overlap_off = true; return RKG_OK; }
+// device's free memory.  Otherwise, or when an allocation fails (what was
+// allocated is released), the sweeps replay on the grid's stream.
+void ensure_overlap(rkg_adjoint* a) {
+    if (a->rstream || a->overlap_off) return;
+    a->overlap_off = true;
+    if (a->nseg < 2) return;
     size_t free_b = 0, total_b = 0;
-    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); return; }
     const size_t hist_b = (size_t)a->K * HIST * sizeof(double);
-    if (ws_bytes(a) + hist_b > free_b / 2) { a->overlap_off = true; return RKG_OK; }
+    if (ws_bytes(a) + hist_b > free_b / 2) return;
     hipError_t e = hipMalloc((void**)&a->ws_alt, ws_bytes(a));
     if (e == hipSuccess) e = hipMalloc((void**)&a->hist_alt, hist_b);
     if (e == hipSuccess) e = hipMemset(a->ws_alt, 0, ws_bytes(a));
     for (int q = 0; q < 4 && e == hipSuccess; ++q) e = hipEventCreateWithFlags(&a->seg_ev[q], hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->rstream, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        set_error("adjoint workspace allocation failed: %s", hipGetErrorString(e));
-        return RKG_ERR_HIP;
+        (void)hipGetLastError();
+        if (a->ws_alt) (void)hipFree(a->ws_alt);
+        if (a->hist_alt) (void)hipFree(a->hist_alt);
+        for (hipEvent_t& ev : a->seg_ev)
+            if (ev) { (void)hipEventDestroy(ev); ev = nullptr; }
+        a->ws_alt = a->hist_alt = nullptr;
+        a->rstream = nullptr;
+        return;
     }
-    return RKG_OK;
+    a->overlap_off = false;
 }
 
 // Reverse of step k of segment s (global step n0 + k).
@@ -1319,10 +1327,9 @@ int rkg_adjoint_forward(rkg_adjoint* a, double* loss) {
 int rkg_adjoint_backward(rkg_adjoint* a, double loss_scale, int tile_rows) {
     if (!valid(a)) return RKG_ERR_BAD_HANDLE;
     if (!a->have_forward) { set_error("rkg_adjoint_backward before rkg_adjoint_forward"); return RKG_ERR_NOT_READY; }
-    int rc = ensure_overlap(a);   // before the sweep's start event: the first call allocates
-    if (rc) return rc;
+    ensure_overlap(a);   // before the sweep's start event: the first call allocates
     BwdPlan pl;
-    rc = bwd_begin(a, loss_scale, tile_rows, pl);
+    int rc = bwd_begin(a, loss_scale, tile_rows, pl);
     if (rc) return rc;
     if (a->rstream) {
         rc = bwd_overlapped(a, pl);
