@@ -151,7 +151,7 @@ def test_directional_derivatives_against_gpu_forward(coriolis, dt):
 
 def test_checkpoint_interval_does_not_change_gradients():
     from rakali_amd import Grid
-    from rakali_amd.adjoint import GaugeMisfit
+    from rakali_amd.adjoint import GaugeMisfit, backward_many
     nx, ny, steps = 50, 33, 17
     outs = []
     for K in (1, 4, 17):
@@ -166,6 +166,9 @@ def test_checkpoint_interval_does_not_change_gradients():
         # K < steps replays segment s - 1 on a second stream and workspace
         # while segment s is reversed; a second sweep reuses them
         sess.backward()
+        assert np.array_equal(sess.gradient("b"), outs[-1][1]) and np.array_equal(sess.gradient("n"), outs[-1][2])
+        # a batch of one replays each segment on the grid's stream, in turn
+        backward_many([sess])
         assert np.array_equal(sess.gradient("b"), outs[-1][1]) and np.array_equal(sess.gradient("n"), outs[-1][2])
         sess.close()
         g.close()
