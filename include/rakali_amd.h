@@ -348,7 +348,7 @@ int rkg_adjoint_forward(rkg_adjoint *a, double *loss);
 /* gauge eta of every step of the last forward run (n_steps x n_gauges) */
 int rkg_adjoint_gauge_eta(rkg_adjoint *a, double *eta);
 /* gradients of loss_scale * L; tile_rows of the reverse y-face kernel (0:
- * chosen from the grid size so that about 1900 tiles run at once, 4 to 32
+ * chosen from the grid size so that about 2048 tiles run at once, 4 to 32
  * rows).  With two or more segments, the first call also allocates a
  * second segment workspace ((12 K + 3) planes) and a stream of its own when
  * that takes at most half of the free device memory: segment s - 1 is then

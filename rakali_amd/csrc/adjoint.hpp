@@ -201,8 +201,10 @@ __host__ __device__ __forceinline__ void riemann_rev(double g, double hl, double
     }
     if (hl > 0.0 && al > 0.0) lhl += lal * g / (2.0 * al);
     if (hr > 0.0 && ar > 0.0) lhr += lar * g / (2.0 * ar);
-    if (wl) { lnl += lul / hl; ltl += lvl / hl; lhl -= (lul * ul + lvl * vl) / hl; }
-    if (wr) { lnr += lur / hr; ltr += lvr / hr; lhr -= (lur * ur + lvr * vr) / hr; }
+    // u = n / h: the adjoint terms share one reciprocal per side (no branch
+    // of the forward run depends on them)
+    if (wl) { const double r = 1.0 / hl; lnl += lul * r; ltl += lvl * r; lhl -= (lul * ul + lvl * vl) * r; }
+    if (wr) { const double r = 1.0 / hr; lnr += lur * r; ltr += lvr * r; lhr -= (lur * ur + lvr * vr) * r; }
 }
 
 // ------------------------------------------------------------------ faces
@@ -254,8 +256,8 @@ __host__ __device__ __forceinline__ void face_rev(double g, const FState& L, dou
     const double lunl = a1 * hl, lutl = a2 * hl, lunr = b1 * hr, lutr = b2 * hr;
     const double lul = YDIR ? lutl : lunl, lvl = YDIR ? lunl : lutl;
     const double lur = YDIR ? lutr : lunr, lvr = YDIR ? lunr : lutr;
-    if (wl) { lL.hu += lul / L.h; lL.hv += lvl / L.h; lL.h -= (lul * u_l + lvl * v_l) / L.h; }
-    if (wr) { lR.hu += lur / R.h; lR.hv += lvr / R.h; lR.h -= (lur * u_r + lvr * v_r) / R.h; }
+    if (wl) { const double r = 1.0 / L.h; lL.hu += lul * r; lL.hv += lvl * r; lL.h -= (lul * u_l + lvl * v_l) * r; }
+    if (wr) { const double r = 1.0 / R.h; lR.hu += lur * r; lR.hv += lvr * r; lR.h -= (lur * u_r + lvr * v_r) * r; }
     double lbmax = 0.0;
     if (pl > 0.0) { lL.h += lhl; lbL += lhl; lbmax -= lhl; }
     if (pr > 0.0) { lR.h += lhr; lbR += lhr; lbmax -= lhr; }
