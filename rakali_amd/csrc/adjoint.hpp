@@ -199,12 +199,19 @@ __host__ __device__ __forceinline__ void riemann_rev(double g, double hl, double
         if ((ul - al) < (ur - ar)) { lul += lsl; lal -= lsl; } else { lur += lsl; lar -= lsl; }
         if ((ul + al) > (ur + ar)) { lul += lsr; lal += lsr; } else { lur += lsr; lar += lsr; }
     }
-    if (hl > 0.0 && al > 0.0) lhl += lal * g / (2.0 * al);
-    if (hr > 0.0 && ar > 0.0) lhr += lar * g / (2.0 * ar);
-    // u = n / h: the adjoint terms share one reciprocal per side (no branch
-    // of the forward run depends on them)
-    if (wl) { const double r = 1.0 / hl; lnl += lul * r; ltl += lvl * r; lhl -= (lul * ul + lvl * vl) * r; }
-    if (wr) { const double r = 1.0 / hr; lnr += lur * r; ltr += lvr * r; lhr -= (lur * ur + lvr * vr) * r; }
+    // a = sqrt(g h): d a / d h = g / (2 a) = a / (2 h); u = n / h.  These
+    // adjoint terms share one reciprocal of h per side (no branch of the
+    // forward run depends on them); a wet side (h > DRY_TOLERANCE) has a > 0.
+    if (hl > 0.0 && al > 0.0) {
+        const double r = 1.0 / hl;
+        lhl += lal * (0.5 * al) * r;
+        if (wl) { lnl += lul * r; ltl += lvl * r; lhl -= (lul * ul + lvl * vl) * r; }
+    }
+    if (hr > 0.0 && ar > 0.0) {
+        const double r = 1.0 / hr;
+        lhr += lar * (0.5 * ar) * r;
+        if (wr) { lnr += lur * r; ltr += lvr * r; lhr -= (lur * ur + lvr * vr) * r; }
+    }
 }
 
 // ------------------------------------------------------------------ faces
