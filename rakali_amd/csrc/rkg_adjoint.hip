@@ -263,7 +263,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
     block_range(d.ny, a.ty, sy, j0, j1);
     const bool xin = c0 + lane < d.nx;
     const int x = xin ? c0 + lane : d.nx - 1;   // lanes past the edge repeat the last column, unwritten
-    const double eta = tidal_eta(d, tau);
+    // rows j0 - 3 .. j1 + 2 are read; only ghost rows need the tide (whose
+    // correctly rounded sine is a long dependent chain)
+    const double eta = (j0 < 3 || j1 + 2 >= d.ny) ? tidal_eta(d, tau) : 0.0;
     const double g = d.g;
     const double rdy = 1.0 / d.dy;
     const long long P = d.plane;
@@ -382,7 +384,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
     const int x = c0 - 2 + lane;
     const bool own_col = x >= c0 && x < c1;
     const bool edge_tile = c0 - 2 < 0 || c0 + WAVE - 3 >= d.nx;
-    const double eta = tidal_eta(d, tau);
+    // columns c0 - 3 .. c0 + 62 are read (lanes 0 and 63 load one more);
+    // only ghost columns need the tide
+    const double eta = (c0 < 3 || c0 + 62 >= d.nx) ? tidal_eta(d, tau) : 0.0;
     const double g = d.g;
     const double rdx = 1.0 / d.dx;
     const long long P = d.plane;
