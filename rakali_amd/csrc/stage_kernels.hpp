@@ -70,6 +70,8 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     s.alpha = 0.0; s.a_prev = 1.0;
     s.blend = false; s.first = false; s.last = false;
     s.radiate = false; s.prev_h = nullptr; s.bc_dt = 0.0;
+    s.eta_tidal = 0.0;
+    double t_tide;   // bc%time of the stage; the tide is evaluated by edge tiles only
     if (STAGE == 1 || (STAGE == 3 && gs.first)) {
         const CtrlSlot cp = d.ctrl[p];
         const bool active = (mode == MODE_STEP) || (cp.t < tlim - 1.0e-10);
@@ -106,7 +108,7 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
             s.oh = gs.oh; s.ohu = gs.ohu; s.ohv = gs.ohv;
             s.dt = gs.beta * dt;
         }
-        s.eta_tidal = tidal_eta(d, cp.t);  // stage abscissa c_1 = 0
+        t_tide = cp.t;  // stage abscissa c_1 = 0
     } else if (STAGE == 3) {
         const CtrlSlot cp = d.ctrl[p];
         const CtrlSlot cq = d.ctrl[1 - p];
@@ -114,7 +116,7 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
         s.ih = gs.ih; s.ihu = gs.ihu; s.ihv = gs.ihv;
         s.oh = gs.oh; s.ohu = gs.ohu; s.ohv = gs.ohv;
         s.dt = gs.beta * cq.dt;
-        s.eta_tidal = tidal_eta(d, cp.t + gs.c * cq.dt);  // bc%time = t_base + c_i dt
+        t_tide = cp.t + gs.c * cq.dt;  // bc%time = t_base + c_i dt
         s.bc_dt = cq.dt;
         s.radiate = true;       // against the previous stage's input
         s.prev_h = gs.ph;
@@ -124,7 +126,7 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
         s.ih = d.bh; s.ihu = d.bhu; s.ihv = d.bhv;
         s.oh = d.o2h; s.ohu = d.o2hu; s.ohv = d.o2hv;
         s.dt = cq.dt;
-        s.eta_tidal = tidal_eta(d, cq.t);  // bc%time = t + dt
+        t_tide = cq.t;  // bc%time = t + dt
         s.bc_dt = cq.dt;
         s.radiate = true;       // against U^n, the input of stage 1
         s.prev_h = d.ah;
@@ -143,10 +145,14 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     s.row0 = seg * d.ty;
     s.row1 = min(s.row0 + d.ty, d.ny);
     double smax, fsum, ssum;
-    if (tile_is_interior(d, strip, seg))
+    if (tile_is_interior(d, strip, seg)) {
+        // no ghost cells: the tide (a correctly rounded sine, a long
+        // dependent chain) is not needed
         smax = process_tile<STAGE, HLLC, false, NF, REC, FEAT>(d, s, L, strip, seg, fsum, ssum);
-    else
+    } else {
+        s.eta_tidal = tidal_eta(d, t_tide);
         smax = process_tile<STAGE, HLLC, true, NF, REC, FEAT>(d, s, L, strip, seg, fsum, ssum);
+    }
     if (STAGE == 2 || STAGE == 3) {
         smax = wave_max(smax);
         if (threadIdx.x == 0 && smax > 0.0)
