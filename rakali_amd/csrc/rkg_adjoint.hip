@@ -1095,16 +1095,16 @@ struct BwdPlan {
 int bwd_begin(rkg_adjoint* a, double loss_scale, int tile_rows, BwdPlan& pl) {
     if (!a->have_forward) { set_error("rkg_adjoint_backward before rkg_adjoint_forward"); return RKG_ERR_NOT_READY; }
     rkg_grid* g = a->g;
-    // rows per workgroup: k_adj_y about 2048 workgroups (two waves on each of
-    // the 1024 SIMDs), k_adj_x about 1900 (a little under, so that they all
-    // run at once); measured on the 1024^2 basin, 8 rows of k_adj_y (2048
-    // workgroups) beat 9 (1821) by 1 %
+    // rows per workgroup: about 2048 workgroups of k_adj_y and of k_adj_x
+    // (two waves on each of the 1024 SIMDs); on the 1024^2 basin 8 rows of
+    // k_adj_y (2048 workgroups) beat 9 (1821) by 1 %, and 9 rows of k_adj_x
+    // (2048) beat 10 (1844) by about 1 %
     const long long ny = g->cfg.ny;
     const long long ystrips = (g->cfg.nx + WAVE - 1) / WAVE, xstrips = (g->cfg.nx + AW - 1) / AW;
     if (tile_rows <= 0) tile_rows = (int)std::min<long long>(32, std::max<long long>(4, (ny * ystrips + 2047) / 2048));
     if (tile_rows < 4 || tile_rows > 256) { set_error("tile_rows must lie in [4, 256]"); return RKG_ERR_RANGE; }
     pl.tile_rows = tile_rows;
-    pl.xrows = (int)std::min<long long>(32, std::max<long long>(1, (ny * xstrips + 1899) / 1900));
+    pl.xrows = (int)std::min<long long>(32, std::max<long long>(1, (ny * xstrips + 2047) / 2048));
     pl.nb = body_blocks(g);
     pl.cfl_dt = g->cfg.dt_fixed <= 0.0;
     pl.nargmax = pl.cfl_dt ? pl.nb : 0;   // argmax partials, one per body workgroup of stage 1
