@@ -10,6 +10,7 @@ Arrays are numpy float64 in Fortran order with shape ``(nx + 2*ng, ny + 2*ng)``
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -83,37 +84,56 @@ def lib():
     if _lib is None:
         if not _LIB_PATH.exists():
             build()
-        L = ctypes.CDLL(str(_LIB_PATH))
-        P = ctypes.POINTER(Params)
-        dp = ctypes.POINTER(ctypes.c_double)
-        L.orc_default_params.argtypes = [P]
-        L.orc_compute_flux.argtypes = [P] + [dp] * 7 + [ctypes.c_int]
-        L.orc_fused_stage1.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
-        L.orc_fused_stage2.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
-        L.orc_apply_sponge.argtypes = [P, dp, dp, dp, ctypes.c_double]
-        L.orc_fused_stage_generic.argtypes = [P] + [dp] * 6 + [ctypes.c_double]
-        ip = ctypes.c_void_p
-        L.orc_gauges_setup.argtypes = [P, ctypes.c_int] + [ip] * 7
-        L.orc_gauges_setup.restype = ctypes.c_int
-        L.orc_gauges_sample.argtypes = [P, dp, dp, dp, dp, ctypes.c_int] + [ip] * 7
-        L.orc_compute_timestep.argtypes = [P, dp, dp, dp]
-        L.orc_compute_timestep.restype = ctypes.c_double
-        L.orc_tidal_eta.argtypes = [P, ctypes.c_double]
-        L.orc_tidal_eta.restype = ctypes.c_double
-        L.orc_apply_bc.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
-        L.orc_apply_bc_dt.argtypes = [P, ctypes.c_double, ctypes.c_double, dp, dp, dp, dp]
-        L.orc_chapman_seed.argtypes = [P, dp, dp]
-        L.orc_fill_bathymetry_ghosts.argtypes = [P, dp]
-        L.orc_init_still_water.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
-        L.orc_total_mass.argtypes = [P, dp]
-        L.orc_total_mass.restype = ctypes.c_double
-        L.orc_run.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(Clock),
-                              ctypes.c_int, ctypes.c_longlong, ctypes.c_double]
-        L.orc_run.restype = ctypes.c_longlong
-        L.orc_set_threads.argtypes = [ctypes.c_int]
-        L.orc_get_threads.restype = ctypes.c_int
-        _lib = L
+        _lib = bind(_LIB_PATH)
     return _lib
+
+
+def bind(path) -> ctypes.CDLL:
+    """Loads a build of rk_oracle.c and declares its entry points (the
+    mutation tests bind deliberately broken copies through this)."""
+    L = ctypes.CDLL(str(path))
+    P = ctypes.POINTER(Params)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.orc_default_params.argtypes = [P]
+    L.orc_compute_flux.argtypes = [P] + [dp] * 7 + [ctypes.c_int]
+    L.orc_fused_stage1.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
+    L.orc_fused_stage2.argtypes = [P] + [dp] * 9 + [ctypes.c_double]
+    L.orc_apply_sponge.argtypes = [P, dp, dp, dp, ctypes.c_double]
+    L.orc_fused_stage_generic.argtypes = [P] + [dp] * 6 + [ctypes.c_double]
+    ip = ctypes.c_void_p
+    L.orc_gauges_setup.argtypes = [P, ctypes.c_int] + [ip] * 7
+    L.orc_gauges_setup.restype = ctypes.c_int
+    L.orc_gauges_sample.argtypes = [P, dp, dp, dp, dp, ctypes.c_int] + [ip] * 7
+    L.orc_compute_timestep.argtypes = [P, dp, dp, dp]
+    L.orc_compute_timestep.restype = ctypes.c_double
+    L.orc_tidal_eta.argtypes = [P, ctypes.c_double]
+    L.orc_tidal_eta.restype = ctypes.c_double
+    L.orc_apply_bc.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
+    L.orc_apply_bc_dt.argtypes = [P, ctypes.c_double, ctypes.c_double, dp, dp, dp, dp]
+    L.orc_chapman_seed.argtypes = [P, dp, dp]
+    L.orc_fill_bathymetry_ghosts.argtypes = [P, dp]
+    L.orc_init_still_water.argtypes = [P, ctypes.c_double, dp, dp, dp, dp]
+    L.orc_total_mass.argtypes = [P, dp]
+    L.orc_total_mass.restype = ctypes.c_double
+    L.orc_run.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(Clock),
+                          ctypes.c_int, ctypes.c_longlong, ctypes.c_double]
+    L.orc_run.restype = ctypes.c_longlong
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_get_threads.restype = ctypes.c_int
+    return L
+
+
+@contextlib.contextmanager
+def using(path):
+    """Routes every oracle call to the build at `path` while the context is
+    open (test infrastructure: the mutation tests)."""
+    global _lib
+    saved = _lib
+    _lib = bind(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
 
 
 def _ptr(a: np.ndarray):
