@@ -289,8 +289,7 @@ def run_ensemble(n_members: int, steps: int, warmup: int = 2):
     crc_loop = 0
     for g in grids:
         crc_loop = zlib.crc32(g.get_field("h").tobytes(), crc_loop)
-    # grids[0] owns the shared stream: close it last
-    for g in reversed(grids):
+    for g in grids:   # the library destroys the shared stream with the last grid using it
         g.close()
 
     return {

@@ -135,10 +135,15 @@ int rkg_config_default(rkg_config *cfg);
 
 /* Allocates device state for one grid on the current HIP device.  `stream`
  * is a hipStream_t to launch on, or NULL for a library-owned stream.  A
- * caller-provided stream (including one obtained from another grid with
- * rkg_get_stream) must outlive every grid that uses it. */
+ * stream obtained from another grid with rkg_get_stream is shared: the grids
+ * may be destroyed in any order and the stream goes with the last of them.
+ * Any other caller-provided stream is borrowed and must outlive every grid
+ * that uses it. */
 int rkg_create(const rkg_config *cfg, void *stream, rkg_grid **out);
-/* Frees the grid and nulls *g; a NULL or already-destroyed handle is a no-op. */
+/* Frees the grid and nulls *g; a NULL or already-destroyed handle is a no-op.
+ * If adjoint sessions of the grid are still alive, its device state is kept
+ * until the last of them is destroyed; those sessions then accept only
+ * rkg_adjoint_destroy (every other call returns RKG_ERR_BAD_HANDLE). */
 void rkg_destroy(rkg_grid **g);
 
 /* Host <-> device copies of one interior field (see the header comment for
@@ -340,6 +345,7 @@ int rkg_adjoint_create(rkg_grid *g, int n_steps, int checkpoint_interval, int n_
  * whose gauge series is output only. */
 int rkg_adjoint_create_stations(rkg_grid *g, int n_steps, int checkpoint_interval, int n_stations,
                                 const double *x, const double *y, const double *targets, rkg_adjoint **out);
+/* Frees the session and nulls *a; releases the session's hold on its grid. */
 void rkg_adjoint_destroy(rkg_adjoint **a);
 int rkg_adjoint_set_targets(rkg_adjoint *a, const double *targets);
 /* n_steps steps from the grid's current state; returns L; the grid is

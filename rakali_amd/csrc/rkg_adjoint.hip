@@ -745,7 +745,17 @@ struct rkg_adjoint {
 
 namespace {
 
-bool valid(const rkg_adjoint* a) { return a != nullptr && a->magic == ADJ_MAGIC; }
+// A live session on a live grid (the grid may have been destroyed by its
+// caller while the session still holds its device state: then only
+// rkg_adjoint_destroy is accepted).
+bool valid(const rkg_adjoint* a) {
+    if (a == nullptr || a->magic != ADJ_MAGIC) return false;
+    if (!grid_valid(a->g)) {
+        set_error("the grid of this adjoint session has been destroyed");
+        return false;
+    }
+    return true;
+}
 
 size_t P(const rkg_adjoint* a) { return a->g->plane; }
 double* wsA(rkg_adjoint* a, int k) { return a->ws + (size_t)k * 3 * P(a); }
@@ -834,6 +844,7 @@ int create_session(rkg_grid* g, int nsteps, int K, int ng, const int* gauge_i, c
     memset((void*)a, 0, sizeof *a);
     a->ckpt_parity = std::vector<int>();
     a->g = g;
+    grid_retain(g);   // released by rkg_adjoint_destroy
     a->nsteps = nsteps;
     a->K = std::min(K, nsteps);
     a->nseg = (nsteps + a->K - 1) / a->K;
@@ -981,9 +992,11 @@ void rkg_adjoint_destroy(rkg_adjoint** pa) {
     if (a->loss_host) (void)hipHostFree(a->loss_host);
     if (a->ev[0]) (void)hipEventDestroy(a->ev[0]);
     if (a->ev[1]) (void)hipEventDestroy(a->ev[1]);
+    rkg_grid* g = a->g;
     a->magic = ADJ_DEAD;
     delete a;
     *pa = nullptr;
+    if (g) grid_release(g);
 }
 
 int rkg_adjoint_set_targets(rkg_adjoint* a, const double* targets) {

@@ -16,7 +16,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/rakali_amd.h"
@@ -42,7 +44,39 @@ void rk::set_error(const char* fmt, ...) {
 namespace {
 
 constexpr unsigned GRID_MAGIC = GRID_MAGIC_LIVE;
+constexpr unsigned GRID_CLOSED = 0x72616B43u;  // 'rakC': destroyed by its caller, still held by a session
 constexpr unsigned GRID_DEAD = 0xDEADBEEFu;
+
+// Streams that rkg_create made, with the number of grids launching on each.
+// A grid created with such a stream (from rkg_get_stream of another grid)
+// shares its ownership, so the grids may be destroyed in any order and the
+// stream is destroyed with the last of them.  Streams the caller made are
+// not in the table and are never destroyed by the library.
+std::mutex g_stream_mu;
+std::unordered_map<hipStream_t, int> g_stream_refs;
+
+bool stream_retain(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    auto it = g_stream_refs.find(s);
+    if (it == g_stream_refs.end()) return false;
+    ++it->second;
+    return true;
+}
+
+void stream_register(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    g_stream_refs[s] = 1;
+}
+
+// True when the caller held the last reference and must destroy the stream.
+bool stream_release(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    auto it = g_stream_refs.find(s);
+    if (it == g_stream_refs.end()) return false;
+    if (--it->second > 0) return false;
+    g_stream_refs.erase(it);
+    return true;
+}
 
 // Plain copy, 16 bytes per lane per iteration: the bandwidth reference
 // for the stage kernels and the calibration of the HBM traffic counters.
@@ -291,6 +325,31 @@ int check_config(const rkg_config* c) {
 
 }  // namespace
 
+int rk::validate_config(const rkg_config* c) { return check_config(c); }
+
+void rk::grid_retain(rkg_grid* g) { ++g->refs; }
+
+void rk::grid_release(rkg_grid* g) {
+    if (--g->refs > 0) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->buf) (void)hipFree(g->buf);
+    if (g->save) (void)hipFree(g->save);
+    if (g->nfield) (void)hipFree(g->nfield);
+    if (g->cbuf) (void)hipFree(g->cbuf);
+    if (g->mpart) (void)hipFree(g->mpart);
+    if (g->frc) (void)hipFree(g->frc);
+    gauges_free(g);
+    if (g->ctrl) (void)hipFree(g->ctrl);
+    if (g->mass_buf) (void)hipFree(g->mass_buf);
+    if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
+    if (g->mass_host) (void)hipHostFree(g->mass_host);
+    for (int k = 0; k < 3; ++k)
+        if (g->ev[k]) (void)hipEventDestroy(g->ev[k]);
+    if (g->own_stream && g->stream && stream_release(g->stream)) (void)hipStreamDestroy(g->stream);
+    g->magic = GRID_DEAD;
+    delete g;
+}
+
 extern "C" {
 
 int rkg_config_default(rkg_config* c) {
@@ -319,12 +378,14 @@ int rkg_create(const rkg_config* cfg, void* stream, rkg_grid** out) {
     g->ty = 0;    // chosen from the grid size (tile_rows_of)
     g->scheme = RKG_TIME_RK2;
     HIP_TRY(hipGetDevice(&g->device));
+    g->refs = 1;
     if (stream) {
         g->stream = (hipStream_t)stream;
-        g->own_stream = false;
+        g->own_stream = stream_retain(g->stream);   // shared with the grid that made it
     } else {
         hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
         if (e != hipSuccess) { set_error("hipStreamCreate: %s", hipGetErrorString(e)); delete g; return RKG_ERR_HIP; }
+        stream_register(g->stream);
         g->own_stream = true;
     }
     long long pitch = ((long long)cfg->nx + 15) / 16 * 16;
@@ -356,26 +417,15 @@ int rkg_create(const rkg_config* cfg, void* stream, rkg_grid** out) {
 void rkg_destroy(rkg_grid** pg) {
     if (!pg || !*pg) return;
     rkg_grid* g = *pg;
-    if (g->magic != GRID_MAGIC) { *pg = nullptr; return; }
-    if (g->stream) (void)hipStreamSynchronize(g->stream);
-    if (g->buf) (void)hipFree(g->buf);
-    if (g->save) (void)hipFree(g->save);
-    if (g->nfield) (void)hipFree(g->nfield);
-    if (g->cbuf) (void)hipFree(g->cbuf);
-    if (g->mpart) (void)hipFree(g->mpart);
-    if (g->frc) (void)hipFree(g->frc);
-    gauges_free(g);
-    if (g->ctrl) (void)hipFree(g->ctrl);
-    if (g->mass_buf) (void)hipFree(g->mass_buf);
-    if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
-    if (g->mass_host) (void)hipHostFree(g->mass_host);
-    for (int k = 0; k < 3; ++k)
-        if (g->ev[k]) (void)hipEventDestroy(g->ev[k]);
-    if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
-    g->magic = GRID_DEAD;
-    delete g;
     *pg = nullptr;
+    if (g->magic != GRID_MAGIC) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    // API calls on the handle fail from here on; an adjoint session still
+    // attached keeps the device state until it is destroyed itself
+    g->magic = GRID_CLOSED;
+    grid_release(g);
 }
+
 
 static int copy_field(rkg_grid* g, int field, double* host_or_dev, int ld, bool upload, hipMemcpyKind kind) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;

@@ -26,7 +26,8 @@ struct rkg_grid {
     rkg_config cfg;
     int device;
     hipStream_t stream;
-    bool own_stream;
+    bool own_stream;     // the stream was made by rkg_create (this grid holds a reference to it)
+    int refs;            // 1 for the caller's handle + 1 per attached adjoint session
     long long pitch;
     size_t plane;        // elements per field plane
     double* buf;         // 7 planes: A(h,hu,hv) B(h,hu,hv) bed
@@ -62,6 +63,17 @@ namespace rk {
 
 constexpr unsigned GRID_MAGIC_LIVE = 0x72616B47u;  // 'rakG', rkg_grid::magic of a live grid
 inline bool grid_valid(const rkg_grid* g) { return g != nullptr && g->magic == GRID_MAGIC_LIVE; }
+
+// Holds / drops a reference to the grid's device state: an adjoint session
+// retains its grid, so destroying the grid first leaves the session's
+// buffers valid until the session is destroyed too (the grid is freed with
+// its last reference).
+void grid_retain(rkg_grid* g);
+void grid_release(rkg_grid* g);
+
+// The checks rkg_create applies to a configuration (sizes, spacing, tags);
+// RKG_OK or an error code with the message set.
+int validate_config(const rkg_config* c);
 
 inline double* plane_ptr(rkg_grid* g, int k) { return g->buf + (size_t)k * g->plane; }
 
@@ -124,10 +136,16 @@ inline double* field_ptr(rkg_grid* g, int field) {
 // at least 8192 tiles of 16 rows (several waves of tiles per wave slot, so
 // longer tiles waste less on halos), else 8 (smaller grids run one tile
 // per slot, and shorter tiles finish sooner).
+// Rows per stage tile by grid size: 16 once a grid has 8192 tiles of 16
+// rows, else 8 (a small grid needs more, shorter tiles to fill the chip).
+inline int default_tile_rows(int nx, int ny) {
+    const long long strips = (nx + 63) / 64;
+    return strips * ((ny + 15) / 16) >= 8192 ? 16 : 8;
+}
+
 inline int tile_rows_of(const rkg_grid* g) {
     if (g->ty > 0) return g->ty;
-    const long long strips = (g->cfg.nx + 63) / 64;
-    return strips * ((g->cfg.ny + 15) / 16) >= 8192 ? 16 : 8;
+    return default_tile_rows(g->cfg.nx, g->cfg.ny);
 }
 
 // apply_sponge does something (rki_kernel_sponge.F90:42-47 and a sponge edge).

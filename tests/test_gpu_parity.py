@@ -207,24 +207,31 @@ def test_mass_conservation_closed_box():
     assert rel <= 1e-13
 
 
-def test_spread_against_platform_libm():
+@pytest.mark.parametrize("case", ["tidal_manning", "dam_break"])
+def test_spread_against_platform_libm(case):
     """Against an oracle that calls glibc's pow/sin/cos (as an x86 build of
-    the reference does), the GPU differs only through last-ulp libm
-    rounding: measured and bounded here (dam break over bumps, 100 steps)."""
+    the reference does), the GPU may differ only through last-ulp libm
+    rounding.  On a tide-plus-Manning crop of the 4096^2 recipe (the tide's
+    sine and the friction's h^(1/3) both run every stage) and on the dam
+    break over bumps, 100 steps: per-cell relative error at most 1e-12,
+    the bar for agreement with the reference (measured: 0)."""
     import oracle as O
+    from conftest import field_errors
     from rakali_amd import scenarios as S
-    scn = S.dam_break_bumps()
+    scn = S.tidal_bumps(nx=512, ny=384, steps=100) if case == "tidal_manning" else S.dam_break_bumps()
+    assert scn.grid_kw["manning_n"] > 0.0
     O.lib().orc_set_libm(1)
     try:
         got, _ = gpu_run(scn, 100)
         _, ref = run_oracle(scn, 100)
     finally:
         O.lib().orc_set_libm(0)
-    errs = rel_errors(got, ref)
-    absd = {k: float(np.abs(got[k] - ref[k]).max()) for k in ref}
-    print(f"[glibc-spread] rel={errs} abs={absd}", flush=True)
+    errs = field_errors(got, ref)
+    nbits = {k: int(np.count_nonzero(got[k] != ref[k])) for k in ref}
+    print(f"[glibc-spread] {case}: (max abs, max per-cell rel) {errs}, cells not bitwise {nbits}", flush=True)
     for k in ref:
-        assert absd[k] <= 1e-10 * max(1.0, float(np.abs(ref[k]).max()))
+        assert errs[k][1] <= 1e-12, k
+    assert np.any(got["hu"] != 0.0)
 
 
 def test_errors_and_lifecycle():
