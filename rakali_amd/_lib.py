@@ -138,11 +138,29 @@ SIGNATURES = {
 }
 
 
+def share_hip_runtime():
+    """Load PyTorch's HIP runtime first when PyTorch is installed.
+
+    PyTorch-ROCm ships its own libamdhip64 and libhsa-runtime64 (SONAME
+    libamdhip64.so.7, opened by file name libamdhip64.so from torch/lib).
+    If this library were loaded first, it would bring the system ROCm
+    runtime with the same SONAME and PyTorch would then load a second copy
+    of each; two HSA runtimes in one process do not share the GPU, and
+    PyTorch's finds none.  Loaded after PyTorch, this library binds to the
+    runtime already mapped (the dynamic loader matches the SONAME), so
+    device pointers and streams pass freely between the two."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """Load the HIP library (raises if it has not been built)."""
     global _lib
     if _lib is not None:
         return _lib
+    share_hip_runtime()
     if not LIB_PATH.exists():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -m rakali_amd.build` "

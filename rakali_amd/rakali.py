@@ -105,6 +105,8 @@ _lib = None
 def _get_lib():
     global _lib
     if _lib is None:
+        from ._lib import share_hip_runtime
+        share_hip_runtime()   # one HIP runtime for this library and PyTorch
         lib = ctypes.CDLL(_find_library())
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(lib, name)

@@ -17,6 +17,8 @@ def build(outdir: Path) -> ctypes.CDLL:
                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                     "-I", str(ROOT / "rakali_amd" / "csrc"), "-I", str(ROOT / "include"),
                     str(ROOT / "tests" / "adjoint_ref.hip"), "-o", str(so)], check=True)
+    from rakali_amd._lib import share_hip_runtime
+    share_hip_runtime()   # bind to the HIP runtime PyTorch uses, as the library does
     return ctypes.CDLL(str(so))
 
 
