@@ -165,6 +165,25 @@ int rkg_step(rkg_grid *g, int n);
  * semantics).  Synchronous; *steps_taken may be NULL. */
 int rkg_advance(rkg_grid *g, double target, long long *steps_taken);
 
+/* rkg_step without a grid and without allocating: n SSP-RK2 steps on
+ * state planes the caller owns.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  h, hu, hv (in
+ * place) and b are device planes of the cfg->nx x cfg->ny interior, row
+ * pitch `pitch` >= nx elements; *t, *dt, *step_count hold the clock on
+ * entry (as rkg_get_clock returns it) and the clock after the steps on
+ * return.  Results equal rkg_step's on a grid holding the same state and
+ * clock, bit for bit (no mass budget, RK save, forcing, sponge or Manning
+ * field).  The intermediates (U* and the control slots) live in a fixed
+ * device array of 256 B + 24 B x RKG_NOALLOC_MAX_CELLS per GPU, held until
+ * the process ends; pitch x ny above RKG_NOALLOC_MAX_CELLS returns -2.
+ * Calls on one GPU share that array: calls that overlap on one GPU race.
+ * Launches on `stream` (NULL: the null stream) and synchronizes it to
+ * return the clock. */
+#define RKG_NOALLOC_MAX_CELLS 2097152
+int rkg_step_noalloc(const rkg_config *cfg, double *h, double *hu, double *hv, const double *b,
+                     long long pitch, int n, double *t, double *dt, long long *step_count, void *stream);
+
 /* Clock: time, dt of the last step, and step count (synchronizes). */
 int rkg_get_clock(rkg_grid *g, double *t, double *dt, long long *step_count);
 /* Sets the time and step count (e.g. after a restart). */

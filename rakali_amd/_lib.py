@@ -20,6 +20,8 @@ TIME_SCHEMES = {"euler": 1, "rk2": 2, "rk3": 3, "rk43": 4}
 MAX_TC = 10
 
 RKG_OK = 0
+# rkg_step_noalloc's largest pitch x ny (RKG_NOALLOC_MAX_CELLS)
+NOALLOC_MAX_CELLS = 2097152
 ERRORS = {-1: "not ready", -2: "argument out of range", -4: "HIP runtime error",
           -5: "unsupported configuration", -100: "invalid handle"}
 
@@ -76,6 +78,8 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_longlong)]),
     "rkg_step": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_advance": (ctypes.c_int, [_P, ctypes.c_double, ctypes.POINTER(ctypes.c_longlong)]),
+    "rkg_step_noalloc": (ctypes.c_int, [ctypes.POINTER(Config), _P, _P, _P, _P, ctypes.c_longlong, ctypes.c_int,
+                                        _dp, _dp, ctypes.POINTER(ctypes.c_longlong), _P]),
     "rkg_get_clock": (ctypes.c_int, [_P, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "rkg_set_clock": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_longlong]),
     "rkg_total_mass": (ctypes.c_int, [_P, _dp]),

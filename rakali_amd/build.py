@@ -22,7 +22,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "librakali_amd.so"
-SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip", CSRC / "rkg_adjoint.hip", CSRC / "rkg_gauges.hip",
+SOURCES = [CSRC / "rkg_api.hip", CSRC / "rkg_ensemble.hip", CSRC / "rkg_adjoint.hip", CSRC / "rkg_gauges.hip", CSRC / "rkg_noalloc.hip",
            *(CSRC / f"stage_feat{k}.hip" for k in range(8))]
 HEADERS = [CSRC / "swe_core.hpp", CSRC / "swe_kernels.hpp", CSRC / "crmath.hpp", CSRC / "adjoint.hpp",
            CSRC / "stage_kernels.hpp", CSRC / "stage_launch.hpp", CSRC / "stage_feat.inc",

@@ -156,22 +156,14 @@ inline bool sponge_on(const rkg_grid* g) {
     return false;
 }
 
-inline GridDesc make_desc(rkg_grid* g) {
+// The configuration-derived part of a descriptor for a grid of row pitch
+// `pitch` and `ty` rows per tile; the buffer pointers are left null.
+inline GridDesc desc_of_config(const rkg_config& c, long long pitch, int ty) {
     GridDesc d;
     memset(&d, 0, sizeof d);
-    const rkg_config& c = g->cfg;
-    d.nx = c.nx; d.ny = c.ny; d.pitch = g->pitch;
+    d.nx = c.nx; d.ny = c.ny; d.pitch = pitch;
     d.dx = c.dx; d.dy = c.dy;
-    d.ah = plane_ptr(g, 0); d.ahu = plane_ptr(g, 1); d.ahv = plane_ptr(g, 2);
-    d.bh = plane_ptr(g, 3); d.bhu = plane_ptr(g, 4); d.bhv = plane_ptr(g, 5);
-    d.bed = plane_ptr(g, 6);
-    d.o2h = d.ah; d.o2hu = d.ahu; d.o2hv = d.ahv;
-    d.nfield = g->nfield;
-    d.rec = nullptr;
-    d.plane = (long long)g->plane;
-    d.sh = g->save;
-    d.shu = g->save ? g->save + g->plane : nullptr;
-    d.shv = g->save ? g->save + 2 * g->plane : nullptr;
+    d.plane = pitch * (long long)c.ny;
     d.g = c.g; d.cfl = c.cfl; d.manning_n = c.manning_n; d.coriolis_f = c.coriolis_f;
     d.use_hllc = c.use_hllc;
     for (int s = 0; s < 4; ++s) d.bc[s] = c.bc[s];
@@ -184,9 +176,24 @@ inline GridDesc make_desc(rkg_grid* g) {
     d.clamped_elevation = c.clamped_elevation;
     d.dt_fixed = c.dt_fixed; d.t_end = c.t_end;
     d.cfl_interval = c.cfl_interval;
-    d.ty = tile_rows_of(g);
+    d.ty = ty;
     d.nstrips = (c.nx + WAVE - 1) / WAVE;
     d.nsegs = (c.ny + d.ty - 1) / d.ty;
+    d.ssp_coeff = 1.0;
+    return d;
+}
+
+inline GridDesc make_desc(rkg_grid* g) {
+    GridDesc d = desc_of_config(g->cfg, g->pitch, tile_rows_of(g));
+    d.ah = plane_ptr(g, 0); d.ahu = plane_ptr(g, 1); d.ahv = plane_ptr(g, 2);
+    d.bh = plane_ptr(g, 3); d.bhu = plane_ptr(g, 4); d.bhv = plane_ptr(g, 5);
+    d.bed = plane_ptr(g, 6);
+    d.o2h = d.ah; d.o2hu = d.ahu; d.o2hv = d.ahv;
+    d.nfield = g->nfield;
+    d.rec = nullptr;
+    d.sh = g->save;
+    d.shu = g->save ? g->save + g->plane : nullptr;
+    d.shv = g->save ? g->save + 2 * g->plane : nullptr;
     d.ctrl = g->ctrl;
     d.mpart = g->mpart;
     if (sponge_on(g)) {

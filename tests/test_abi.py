@@ -95,3 +95,31 @@ def test_tidal_ensemble_recipe():
         assert 0.5 <= s.grid_kw["tidal_amplitude"] <= 2.0
         assert 0.015 <= s.grid_kw["manning_n"] <= 0.04
         assert s.b.shape == (s.nx, s.ny) and np.array_equal(s.h, np.maximum(-s.b, 0.0))
+
+
+def test_step_noalloc_rejects_bad_arguments_before_touching_the_gpu():
+    """Sizes beyond the fixed array, a pitch below nx and null buffers are
+    refused by the host checks, before any HIP call (so this runs without a
+    GPU; the pointers are never dereferenced)."""
+    from rakali_amd import _lib
+    from rakali_amd.grid import make_config
+    L = _lib.load()
+    buf = (ctypes.c_double * 8)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    t, dt, n = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_longlong(0)
+
+    def call(cfg, pitch, ptr=p, steps=1):
+        return L.rkg_step_noalloc(ctypes.byref(cfg), ptr, ptr, ptr, ptr, pitch, steps,
+                                  ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n), None)
+
+    ny = 1024
+    wide = _lib.NOALLOC_MAX_CELLS // ny + 1
+    assert call(make_config(wide, ny, 10.0, 10.0), wide) == -2
+    assert b"exceeds the maximum" in L.rkg_last_error()
+    assert call(make_config(64, 32, 10.0, 10.0), 63) == -2
+    assert call(make_config(64, 32, 10.0, 10.0), 64, ptr=None) == -2
+    assert call(make_config(64, 32, 10.0, 10.0), 64, steps=-1) == -2
+    nested = make_config(64, 32, 10.0, 10.0)
+    nested.bc[0] = 4
+    assert call(nested, 64) == -5
+    assert (t.value, dt.value, n.value) == (0.0, 0.0, 0)
