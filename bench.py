@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--tile-rows", type=int, default=0, help="0 = library choice by grid size")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU work of the baseline sample (0 disables)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = the CPUs of this process, up to OMP_NUM_THREADS")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the plain-copy bandwidth reference (0 disables)")
     ap.add_argument("--ensemble-members", type=int, default=64,
@@ -536,7 +536,11 @@ def run_cpu_baseline(scn, seconds: float, threads: int):
     steps as fit in about `seconds` (at least one)."""
     import oracle as O
 
-    nthreads = threads or min(16, os.cpu_count() or 1)
+    # every CPU this process may run on, up to the OpenMP thread count the
+    # host sets for one GPU's share of the machine (OMP_NUM_THREADS)
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS") or affinity)
+    nthreads = threads or max(1, min(affinity, share))
     O.set_threads(nthreads)
     kw = dict(scn.grid_kw)
     bc = kw.pop("bc")
@@ -560,9 +564,13 @@ def run_cpu_baseline(scn, seconds: float, threads: int):
         "value": scn.nx * scn.ny * steps / wall,
         "unit": "cell-updates/s",
         "cores": nthreads,
+        "host_cpus": os.cpu_count(),
+        "affinity_cpus": affinity,
         "kind": "port",
         "sample": f"{steps} SSP-RK2 steps of the same {scn.nx}x{scn.ny} run from the same initial "
-                  f"state ({wall:.1f} s of CPU work, {nthreads} OpenMP threads)",
+                  f"state ({wall:.1f} s of CPU work, {nthreads} OpenMP threads on a host of "
+                  f"{os.cpu_count()} CPUs, {affinity} in this process's affinity mask, "
+                  f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')})",
     }
 
 

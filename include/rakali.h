@@ -108,11 +108,13 @@ int rki_set_tidal_constituents(void *handle, const double *amp, const double *ph
 void rki_sync_host(void *handle);
 void rki_sync_from_device(void *handle);
 
-/* rki_api.F90:1410,1436 -- restart files.  The reference writes NetCDF;
- * this build writes a self-describing little-endian binary file (magic
- * "RKRS", version, nx, ny, ng, time, dt, step count, then the full h, hu,
- * hv and b arrays with ghosts).  Load is a phase-1 call (-2 after
- * finalize) and returns -3 on a missing file or a grid mismatch. */
+/* rki_api.F90:1410,1436 -- restart files in restart_write's layout
+ * (rki_output.F90:740-848): classic 64-bit-offset NetCDF with dimensions
+ * x, y, FP64 h, hu, hv, b on (y, x) (interior cells), a scalar time and the
+ * reference's units and global attributes, written and read without the
+ * NetCDF library.  Load also accepts the binary "RKRS" files of earlier
+ * versions of this library.  Load is a phase-1 call (-2 after finalize)
+ * and returns -3 on a missing or unreadable file or a grid mismatch. */
 int rki_save_restart(void *handle, const char *filename, int len);
 int rki_load_restart(void *handle, const char *filename, int len);
 

@@ -91,6 +91,12 @@ class GaugeMisfit:
             self._h, FIELDS[name], out.ctypes.data_as(ctypes.c_void_p), nx, 0))
         return out
 
+    def gradient_device(self, name: str, ptr: int, ld: int) -> None:
+        """The gradient copied device to device into `ptr` ((nx, ny) interior,
+        x fastest, `ld` >= nx elements per row); returns once it is there."""
+        check("rkg_adjoint_gradient", self._L.rkg_adjoint_gradient(
+            self._h, FIELDS[name], ctypes.c_void_p(ptr), int(ld), 1))
+
     def timing(self):
         f, b = ctypes.c_double(), ctypes.c_double()
         check("rkg_adjoint_timing", self._L.rkg_adjoint_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
@@ -153,19 +159,47 @@ def start_still_water(grid, b, n, eta0: float = 0.0) -> None:
     grid.set_clock(0.0, 0)
 
 
+def _start_still_water_device(grid, b, n, eta0: float) -> None:
+    """start_still_water from float64 device tensors of shape (nx, ny):
+    transposed copies (x fastest) go to the grid device to device, and
+    h = max(eta0 - b, 0) is formed on the device with the same IEEE
+    subtraction and the select of fmax0 (eta0 - b > 0 ? eta0 - b : 0)."""
+    import torch
+    nx = grid.nx
+    bt = b.detach().t().contiguous()                       # (ny, nx), x fastest
+    d = eta0 - bt
+    ht = torch.where(d > 0.0, d, torch.zeros_like(d))
+    zt = torch.zeros_like(bt)
+    nt = n.detach().to(torch.float64).t().contiguous()
+    torch.cuda.current_stream(b.device).synchronize()      # the grid copies on its own stream
+    grid.set_field_device("b", bt.data_ptr(), nx)
+    grid.set_field_device("h", ht.data_ptr(), nx)
+    grid.set_field_device("hu", zt.data_ptr(), nx)
+    grid.set_field_device("hv", zt.data_ptr(), nx)
+    grid.use_manning_field(True)
+    grid.set_field_device("n", nt.data_ptr(), nx)
+    grid.set_clock(0.0, 0)
+
+
 def _torch_function():
     import torch
 
     class BathymetryRoughnessLoss(torch.autograd.Function):
         """loss = L(b, n): still water over bed b, per-cell Manning n,
         ``session.n_steps`` steps, gauge misfit against the session's
-        targets.  Gradients come from the discrete adjoint."""
+        targets.  Gradients come from the discrete adjoint.  Tensors on the
+        GPU move device to device (no host copy); CPU tensors go through
+        host memory."""
 
         @staticmethod
         def forward(ctx, b, n, session: GaugeMisfit, eta0: float = 0.0):
-            start_still_water(session.grid, b.detach().cpu().numpy(), n.detach().cpu().numpy(), eta0)
+            on_gpu = b.is_cuda and b.dtype == torch.float64
+            if on_gpu:
+                _start_still_water_device(session.grid, b, n, eta0)
+            else:
+                start_still_water(session.grid, b.detach().cpu().numpy(), n.detach().cpu().numpy(), eta0)
             loss = session.forward()
-            ctx.session, ctx.eta0 = session, eta0
+            ctx.session, ctx.eta0, ctx.on_gpu = session, eta0, on_gpu
             ctx.dev, ctx.dtype = b.device, b.dtype
             return torch.tensor(loss, dtype=b.dtype, device=b.device)
 
@@ -174,6 +208,15 @@ def _torch_function():
             s = ctx.session
             s.backward(scale=float(grad))
             s.fold_still_water(ctx.eta0)
+            if ctx.on_gpu:
+                nx, ny = s.grid.nx, s.grid.ny
+                out = []
+                for name in ("b", "n"):
+                    t = torch.empty((ny, nx), dtype=torch.float64, device=ctx.dev)
+                    torch.cuda.current_stream(ctx.dev).synchronize()   # t is allocated before the copy
+                    s.gradient_device(name, t.data_ptr(), nx)
+                    out.append(t.t())
+                return out[0], out[1], None, None
             gb = torch.from_numpy(np.ascontiguousarray(s.gradient("b"))).to(device=ctx.dev, dtype=ctx.dtype)
             gn = torch.from_numpy(np.ascontiguousarray(s.gradient("n"))).to(device=ctx.dev, dtype=ctx.dtype)
             return gb, gn, None, None

@@ -87,9 +87,32 @@ __device__ __forceinline__ double rdiv(double n, const Recip& r) {
     return fma(-t, r.y, q0);
 }
 
-// Non-zero and below 2^-300 in magnitude: a value for which some quotient
-// of the step could leave the range where rdiv<false> is exact.
-__device__ __forceinline__ bool tiny(double x) { return fabs(x) < 0x1p-300 && x != 0.0; }
+// Non-zero and outside [2^-300, 2^40] in magnitude, or not finite: a value
+// for which some quotient of the step could leave the range where
+// rdiv<false> is exact (the bound 2^40 also keeps every square root and
+// reciprocal argument of the fast row path finite and in range).
+__device__ __forceinline__ bool tiny(double x) {
+    const double a = fabs(x);
+    return x != 0.0 && !(a >= 0x1p-300 && a <= 0x1p40);
+}
+
+// Square root of a finite x >= 2^-767: the correctly rounded sequence the
+// compiler emits for sqrt (reciprocal square root, Goldschmidt refinement,
+// two fma residual corrections) without its scaling of inputs below
+// 2^-767 and its fixup of 0 and infinity, which are identities on that
+// range.  Same bits as sqrt(x) there.
+__device__ __forceinline__ double sqrt_pos(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r;
+    double h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
 __device__ __forceinline__ bool tiny(const Cell& c) {
     return tiny(c.h) || tiny(c.hu) || tiny(c.hv) || tiny(c.b);
 }
@@ -252,6 +275,129 @@ __device__ __forceinline__ Face face_flux(double g, const FState& L, double bL, 
     return f;
 }
 
+// ---------------------------------------------------------------------
+// The fast path.  Away from wet/dry fronts and shocks a wavefront's faces
+// and cells satisfy the assumptions below; each *_fast function performs
+// exactly the IEEE operations of its general counterpart for the branch
+// those assumptions select, and raises `bad` on a lane where one fails.
+// The callers test the assumptions they can before the work (wave-uniform
+// branches) and redo a piece on the general path when any lane that
+// matters raised `bad`, so results never depend on the path taken.  The
+// assumptions and what they remove:
+//  * every face state and hydrostatic depth exceeds DRY_TOLERANCE: the
+//    velocity quotients need no dry branch, the wave-speed arguments g h
+//    lie above 2^-767 and are finite (the row's inputs are bounded by
+//    tiny()), so sqrt_pos gives sqrt's bits, and the speeds are Davis's;
+//  * s_l < 0 < s_r (the HLL star region) with s_r - s_l in [1e-8, 1e12]:
+//    1/(s_r - s_l) is the shared-reciprocal quotient of 1.0, the same bits
+//    as the division;
+//  * HLLC: |denom_star| > DRY_TOLERANCE and num = 0 or |num| >= 2^-900.
+//    Only the sign of s_star = num / denom_star is used; |denom_star| stays
+//    below 2^83 (depths below 2^41 for inputs bounded by tiny(), and
+//    |u - s| <= 2 (s_r - s_l) <= 2e12), so the quotient does not underflow
+//    to zero and
+//    s_star >= 0 exactly when num is zero or num and denom_star have the
+//    same sign: no division;
+//  * updated cells: h >= DRY_TOLERANCE with h^4 >= DESING_EPS^4 (no clamp,
+//    no KP07) and speed^2 below the Froude backstop by a margin checked
+//    without the division; friction's x^(1/3) below 1e12 and its
+//    numerator g n^2 at least 2^-969, so its quotient is a shared-reciprocal
+//    one.
+
+// riemann_x restricted to the HLL star region, rki_kernel_flux.F90:410-495
+// and 584-684 (Davis speeds, :463-466).
+template <bool HLLC>
+__device__ __forceinline__ void riemann_x_fast(double g, double h_l, double hu_l, double hv_l, double h_r,
+                                               double hu_r, double hv_r, double& f_h, double& f_hu,
+                                               double& f_hv, bool& bad) {
+    const Recip rl = recip(h_l), rr = recip(h_r);
+    const double u_l = rdiv<false>(hu_l, rl), v_l = rdiv<false>(hv_l, rl);
+    const double u_r = rdiv<false>(hu_r, rr), v_r = rdiv<false>(hv_r, rr);
+    const double a_l = sqrt_pos(g * h_l);
+    const double a_r = sqrt_pos(g * h_r);
+    const double s_l = dmin(u_l - a_l, u_r - a_r);
+    const double s_r = dmax(u_l + a_l, u_r + a_r);
+    const double fl_h = hu_l, fl_hu = hu_l * u_l + 0.5 * g * h_l * h_l, fl_hv = hu_l * v_l;
+    const double fr_h = hu_r, fr_hu = hu_r * u_r + 0.5 * g * h_r * h_r, fr_hv = hu_r * v_r;
+    const double span = s_r - s_l;
+    bad = bad || !(s_l < 0.0 && s_r > 0.0 && span >= 1.0e-8 && span <= 1.0e12);
+    const double denom = rdiv<false>(1.0, recip(span));
+    f_h = (s_r * fl_h - s_l * fr_h + s_l * s_r * (h_r - h_l)) * denom;
+    f_hu = (s_r * fl_hu - s_l * fr_hu + s_l * s_r * (hu_r - hu_l)) * denom;
+    if (!HLLC) {
+        f_hv = (s_r * fl_hv - s_l * fr_hv + s_l * s_r * (hv_r - hv_l)) * denom;
+    } else {
+        const double denom_star = h_r * (u_r - s_r) - h_l * (u_l - s_l);
+        const double num = s_l * h_r * (u_r - s_r) - s_r * h_l * (u_l - s_l);
+        bad = bad || !(fabs(denom_star) > DRY_TOLERANCE && (num == 0.0 || fabs(num) >= 0x1p-900));
+        const bool star_nonneg = num == 0.0 || ((num > 0.0) == (denom_star > 0.0));
+        f_hv = star_nonneg ? f_h * v_l : f_h * v_r;
+    }
+}
+
+// riemann_y restricted to the HLL star region, rki_kernel_flux.F90:497-582
+// and 686-783.
+template <bool HLLC>
+__device__ __forceinline__ void riemann_y_fast(double g, double h_b, double hu_b, double hv_b, double h_t,
+                                               double hu_t, double hv_t, double& g_h, double& g_hu,
+                                               double& g_hv, bool& bad) {
+    const Recip rb = recip(h_b), rt = recip(h_t);
+    const double u_b = rdiv<false>(hu_b, rb), v_b = rdiv<false>(hv_b, rb);
+    const double u_t = rdiv<false>(hu_t, rt), v_t = rdiv<false>(hv_t, rt);
+    const double a_b = sqrt_pos(g * h_b);
+    const double a_t = sqrt_pos(g * h_t);
+    const double s_l = dmin(v_b - a_b, v_t - a_t);
+    const double s_r = dmax(v_b + a_b, v_t + a_t);
+    const double gl_h = hv_b, gl_hu = hu_b * v_b, gl_hv = hv_b * v_b + 0.5 * g * h_b * h_b;
+    const double gr_h = hv_t, gr_hu = hu_t * v_t, gr_hv = hv_t * v_t + 0.5 * g * h_t * h_t;
+    const double span = s_r - s_l;
+    bad = bad || !(s_l < 0.0 && s_r > 0.0 && span >= 1.0e-8 && span <= 1.0e12);
+    const double denom = rdiv<false>(1.0, recip(span));
+    g_h = (s_r * gl_h - s_l * gr_h + s_l * s_r * (h_t - h_b)) * denom;
+    g_hv = (s_r * gl_hv - s_l * gr_hv + s_l * s_r * (hv_t - hv_b)) * denom;
+    if (!HLLC) {
+        g_hu = (s_r * gl_hu - s_l * gr_hu + s_l * s_r * (hu_t - hu_b)) * denom;
+    } else {
+        const double denom_star = h_t * (v_t - s_r) - h_b * (v_b - s_l);
+        const double num = s_l * h_t * (v_t - s_r) - s_r * h_b * (v_b - s_l);
+        bad = bad || !(fabs(denom_star) > DRY_TOLERANCE && (num == 0.0 || fabs(num) >= 0x1p-900));
+        const bool star_nonneg = num == 0.0 || ((num > 0.0) == (denom_star > 0.0));
+        g_hu = star_nonneg ? g_h * u_b : g_h * u_t;
+    }
+}
+
+// face_flux<HLLC, YDIR, false> on the fast path where the wavefront allows
+// it: when every lane that matters (`use`) has both face states and both
+// hydrostatic depths wet, the velocities need no dry branch and the Riemann
+// solver is riemann_*_fast; lanes outside its star region then take the
+// general solver on the same arguments.  Otherwise face_flux.
+template <bool HLLC, bool YDIR>
+__device__ __forceinline__ Face face_flux_auto(double g, const FState& L, double bL, const FState& R, double bR,
+                                               bool use) {
+    const double b_max = dmax(bL, bR);
+    const double h_hyd_l = fmax0(L.h + bL - b_max);
+    const double h_hyd_r = fmax0(R.h + bR - b_max);
+    const bool wet = L.h > DRY_TOLERANCE && R.h > DRY_TOLERANCE && h_hyd_l > DRY_TOLERANCE &&
+                     h_hyd_r > DRY_TOLERANCE;
+    if (__any(use && !wet)) return face_flux<HLLC, YDIR, false>(g, L, bL, R, bR);
+    const Recip rl = recip(L.h), rr = recip(R.h);
+    const double u_l = rdiv<false>(L.hu, rl), v_l = rdiv<false>(L.hv, rl);
+    const double u_r = rdiv<false>(R.hu, rr), v_r = rdiv<false>(R.hv, rr);
+    const double qu_l = h_hyd_l * u_l, qv_l = h_hyd_l * v_l, qu_r = h_hyd_r * u_r, qv_r = h_hyd_r * v_r;
+    Face f;
+    bool bad = false;
+    if (YDIR) riemann_y_fast<HLLC>(g, h_hyd_l, qu_l, qv_l, h_hyd_r, qu_r, qv_r, f.fh, f.fhu, f.fhv, bad);
+    else riemann_x_fast<HLLC>(g, h_hyd_l, qu_l, qv_l, h_hyd_r, qu_r, qv_r, f.fh, f.fhu, f.fhv, bad);
+    if (__any(use && bad)) {
+        if (YDIR) riemann_y<HLLC, false>(g, h_hyd_l, qu_l, qv_l, h_hyd_r, qu_r, qv_r, f.fh, f.fhu, f.fhv);
+        else riemann_x<HLLC, false>(g, h_hyd_l, qu_l, qv_l, h_hyd_r, qu_r, qv_r, f.fh, f.fhu, f.fhv);
+    }
+    f.a = h_hyd_r * h_hyd_r - R.h * R.h;
+    f.c = h_hyd_l * h_hyd_l;
+    f.d = L.h * L.h;
+    return f;
+}
+
 // Net flux divergence of one cell from its four faces,
 // rki_kernel_flux.F90:358-369 (rdx, rdy: reciprocals of dx, dy).
 template <bool EXACT>
@@ -347,6 +493,76 @@ __device__ __forceinline__ void stage_body(const SourceTerms& s, double dt, doub
         }
     }
     if (FRC) h = h + dt * f.rain;   // every cell, :286-289
+}
+
+// clamp_desing of the fast row path: the cell is wet (h >= DRY_TOLERANCE),
+// outside the KP07 band (h^4 >= DESING_EPS^4) and below the Froude
+// backstop, so the values pass unchanged.  The backstop test
+// (hu^2 + hv^2) / h^2 > fr_max_sq is decided without the quotient:
+// num < h^2 (fr_max_sq (1 - 2^-50)), rounded, implies num / h^2 < fr_max_sq
+// exactly, hence the rounded quotient is at most fr_max_sq (a double).
+__device__ __forceinline__ void clamp_fast(double g, double h, double hu, double hv, bool& bad) {
+    const double eps4 = 5.062499999999998e-16;
+    const double h4 = pow4(h);
+    const double num = hu * hu + hv * hv;
+    const double fr_max_sq = (FROUDE_BACKSTOP * FROUDE_BACKSTOP * g) * h;
+    const double lim = (h * h) * (fr_max_sq * (1.0 - 0x1p-50));
+    bad = bad || !(h >= DRY_TOLERANCE && h4 >= eps4 && num < lim);
+}
+
+// stage_body of the fast row path (clamp_fast above; friction's quotient
+// by x^(1/3) as a shared-reciprocal one while x^(1/3) <= 1e12 and
+// g n^2 >= 2^-969).
+template <bool NF, bool FRC = false>
+__device__ __forceinline__ void stage_body_fast(const SourceTerms& s, double dt, double fh, double fhu, double fhv,
+                                                double& h, double& hu, double& hv, double n_cell, const Frc& f,
+                                                bool& bad) {
+    h = h + dt * fh;
+    hu = hu + dt * fhu;
+    hv = hv + dt * fhv;
+    clamp_fast(s.g, h, hu, hv, bad);
+    const bool friction = NF ? (n_cell > 0.0) : (s.has_friction != 0);
+    const double n_sq = NF ? n_cell * n_cell : s.n_sq;
+    if (friction) {
+        const Recip rh = recip(h);
+        const double u = rdiv<false>(hu, rh);
+        const double v = rdiv<false>(hv, rh);
+        const double speed = sqrt(u * u + v * v);
+        const double p13 = crm::pow13(h);
+        const double gn2 = s.g * n_sq;
+        bad = bad || !(p13 <= 1.0e12 && gn2 >= 0x1p-969);
+        const double cf = rdiv<false>(gn2, recip(p13));
+        const Recip rd = recip(1.0 + rdiv<false>(dt * cf * speed, rh));
+        hu = rdiv<false>(hu, rd);
+        hv = rdiv<false>(hv, rd);
+    }
+    if (FRC) {
+        hu = hu + dt * f.wx / RHO_WATER;
+        hv = hv + dt * f.wy / RHO_WATER;
+        hu = hu - dt * h * f.dpdx / RHO_WATER;
+        hv = hv - dt * h * f.dpdy / RHO_WATER;
+    }
+    if (s.has_coriolis) {
+        const double hu_old = hu, hv_old = hv;
+        hu = hu_old * s.cosf + hv_old * s.sinf;
+        hv = -hu_old * s.sinf + hv_old * s.cosf;
+    }
+    if (FRC) h = h + dt * f.rain;
+}
+
+// cfl_speed for a cell that passed clamp_fast (h > DRY_TOLERANCE; g h and
+// h^4 + eps^4 are positive and finite, so sqrt_pos gives sqrt's bits).
+__device__ __forceinline__ double cfl_speed_fast(double g, const Recip& rdx, const Recip& rdy, double h, double hu,
+                                                 double hv) {
+    const double eps4 = 5.062499999999998e-16;
+    const double wave_speed = sqrt_pos(g * h);
+    const double h4 = pow4(h);
+    const Recip rd = recip(sqrt_pos(h4 + eps4));
+    const double u_abs = rdiv<false>(fabs(hu) * SQRT2 * h, rd);
+    const double v_abs = rdiv<false>(fabs(hv) * SQRT2 * h, rd);
+    const double sx = rdiv<false>(u_abs + wave_speed, rdx);
+    const double sy = rdiv<false>(v_abs + wave_speed, rdy);
+    return dmax(sx, sy);
 }
 
 // Per-cell CFL speed term, rki_kernel_timestep.F90:56-69; 0 for dry cells.

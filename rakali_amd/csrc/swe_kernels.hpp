@@ -351,9 +351,12 @@ struct ExactTag {
 // evaluated by its lane and the face right of it is the next lane's.  The
 // edge columns come from a per-segment batch (one row per lane) held in
 // LDS.  Every slope and every face of the tile is evaluated once.
-// A row whose stencil holds a tiny non-zero value (see tiny()) anywhere in
-// the wavefront is evaluated with plain divisions; all others with the
-// shared-reciprocal ones; both give the IEEE quotients.
+// A row whose stencil holds a non-zero value outside [2^-300, 2^40] in
+// magnitude, or a non-finite one (see tiny()), anywhere in the wavefront is
+// evaluated with plain divisions; every other row with the shared-reciprocal
+// quotients, its faces and cell updates on the fast path of swe_core.hpp
+// where the whole wavefront meets its assumptions.  All give the same bits
+// where they apply.
 // STAGE 1 writes U* to B; STAGE 2 reads U* and U^n, applies the RK2 average
 // and writes U^{n+1} over A, returning the lane's max CFL speed.
 // apply_sponge (rki_kernel_sponge.F90:16-88) on one interior cell (c, j),
@@ -393,6 +396,8 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     const int nrows = j1 - j0;
     const double g = d.g;
     const long long pitch = d.pitch;
+    // the fast row path's ranges assume a gravity of ordinary size
+    const bool try_fast = g >= 1.0e-3 && g <= 1.0e6;
     double smax = 0.0;
     double facc = 0.0;  // sum of the lane's flux_h, rows in order
     double sacc = 0.0;  // sum of the lane's sponge change of h
@@ -449,33 +454,35 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         t_rnn = tiny(rnn);
     }
     const bool own = c < d.nx;
-    Cell un;  // U^n at (c, j) for the RK2 average
-    un.b = 0.0;
-    if (read_un && own) {
-        const long long k = (long long)j0 * pitch + c;
-        un.h = d.ah[k]; un.hu = d.ahu[k]; un.hv = d.ahv[k];
-    }
     __syncthreads();
 
     for (int j = j0; j < j1; ++j) {
         const int r = j - j0;
         const bool more = j + 1 < j1;
-        Cell nxt, nun;
-        if (more) {
-            nxt = ld<EDGE, FEAT>(d, s, c, j + 3);
-            if (read_un && own) {
-                const long long k = (long long)(j + 1) * pitch + c;
-                nun.h = d.ah[k]; nun.hu = d.ahu[k]; nun.hv = d.ahv[k];
-            }
+        Cell nxt;
+        if (more) nxt = ld<EDGE, FEAT>(d, s, c, j + 3);
+        // U^n at (c, j) for the RK2 average, needed only at the end of the
+        // row (its load hides behind the faces)
+        Cell un;
+        un.h = 0.0; un.hu = 0.0; un.hv = 0.0; un.b = 0.0;
+        if (read_un && own) {
+            const long long k = (long long)j * pitch + c;
+            un.h = d.ah[k]; un.hu = d.ahu[k]; un.hv = d.ahv[k];
         }
         FState tn;
         Face fyt;
         auto row = [&](auto exact) {
             constexpr bool EX = decltype(exact)::value;
+            // faces and cells take the fast path (swe_core.hpp) where the
+            // wavefront allows it; lanes up to the first column past the
+            // grid feed an owned cell through their x-face
+            const bool fast = !EX && try_fast;
+            const bool use_x = c <= d.nx;
             // y: slope of row j+1 and the face between rows j and j+1
             FState bn;
             cell_states(rc, rn, rnn, bn, tn);
-            fyt = face_flux<HLLC, true, EX>(g, tc, rc.b, bn, rn.b);
+            if (fast) fyt = face_flux_auto<HLLC, true>(g, tc, rc.b, bn, rn.b, own);
+            else fyt = face_flux<HLLC, true, EX>(g, tc, rc.b, bn, rn.b);
 
             // x: slope of row j, face left of each cell, face right from the next lane
             Cell xm = shfl_up1(rc);
@@ -486,7 +493,9 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
             cell_states(xm, rc, xp, xlo, xhi);
             FState rprev = shfl_up1(xhi);
             if (lane == 0) rprev = FState{L.ph[0][r], L.ph[1][r], L.ph[2][r]};
-            const Face fxl = face_flux<HLLC, false, EX>(g, rprev, xm.b, xlo, rc.b);
+            Face fxl;
+            if (fast) fxl = face_flux_auto<HLLC, false>(g, rprev, xm.b, xlo, rc.b, use_x);
+            else fxl = face_flux<HLLC, false, EX>(g, rprev, xm.b, xlo, rc.b);
             Face fxr = shfl_down1(fxl);
             if (lane == WAVE - 1)
                 fxr = Face{L.pf[0][r], L.pf[1][r], L.pf[2][r], L.pf[3][r], L.pf[4][r], L.pf[5][r]};
@@ -508,30 +517,56 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                     fr.dpdy = d.fdpy[k];
                     fr.rain = d.frain[k];
                 }
-                if (NF)
-                    stage_body<EX, true, FRC>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, d.nfield[k], fr);
-                else
-                    stage_body<EX, false, FRC>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, 0.0, fr);
+                const double n_cell = NF ? d.nfield[k] : 0.0;
+                bool body_fast = fast;
+                if (fast) {
+                    bool bad = false;
+                    double h1 = h, hu1 = hu, hv1 = hv;
+                    stage_body_fast<NF, FRC>(s.src, s.dt, fh, fhu, fhv, h1, hu1, hv1, n_cell, fr, bad);
+                    if (__any(bad)) body_fast = false;
+                    else { h = h1; hu = hu1; hv = hv1; }
+                }
+                if (!body_fast) stage_body<EX, NF, FRC>(s.src, s.dt, fh, fhu, fhv, h, hu, hv, n_cell, fr);
                 // the sponge follows each fused stage (rki_solver.F90:787,814)
                 if (STAGE == 1 && SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);
+                // U^n with a tiny value in any lane: the rest of the cell
+                // update with plain divisions
+                const bool ex_un = EX || (read_un && __any(tiny(un)));
                 if (STAGE == 2) {
                     // rki_kernel_fused.F90:435-463
                     h = 0.5 * (un.h + h);
                     hu = 0.5 * (un.hu + hu);
                     hv = 0.5 * (un.hv + hv);
-                    clamp_desing<EX>(g, h, hu, hv);
-                    if (SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);
-                    smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
+                    if (ex_un) {
+                        clamp_desing<true>(g, h, hu, hv);
+                        if (SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);
+                        smax = dmax(smax, cfl_speed<true>(g, s.rdx, s.rdy, h, hu, hv));
+                    } else {
+                        bool cell_fast = fast;
+                        if (fast) {
+                            bool bad = false;
+                            clamp_fast(g, h, hu, hv, bad);   // passes the values unchanged
+                            cell_fast = !__any(bad);
+                        }
+                        if (!cell_fast) clamp_desing<false>(g, h, hu, hv);
+                        if (SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);
+                        if (cell_fast && !SPG) smax = dmax(smax, cfl_speed_fast(g, s.rdx, s.rdy, h, hu, hv));
+                        else smax = dmax(smax, cfl_speed<false>(g, s.rdx, s.rdy, h, hu, hv));
+                    }
                 }
                 if (STAGE == 3) {
                     if (s.blend) {  // compute_ssp_combine, rki_kernel_update.F90:183-206
                         h = s.alpha * un.h + s.a_prev * h;
                         hu = s.alpha * un.hu + s.a_prev * hu;
                         hv = s.alpha * un.hv + s.a_prev * hv;
-                        clamp_desing<EX>(g, h, hu, hv);
+                        if (ex_un) clamp_desing<true>(g, h, hu, hv);
+                        else clamp_desing<false>(g, h, hu, hv);
                     }
                     if (SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);   // :919-920, at beta_i dt
-                    if (s.last) smax = dmax(smax, cfl_speed<EX>(g, s.rdx, s.rdy, h, hu, hv));
+                    if (s.last) {
+                        if (ex_un) smax = dmax(smax, cfl_speed<true>(g, s.rdx, s.rdy, h, hu, hv));
+                        else smax = dmax(smax, cfl_speed<false>(g, s.rdx, s.rdy, h, hu, hv));
+                    }
                 }
                 s.oh[k] = h; s.ohu[k] = hu; s.ohv[k] = hv;
                 if ((STAGE == 1 || (STAGE == 3 && s.first)) && d.sh != nullptr) {
@@ -540,7 +575,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                 }
             }
         };
-        const bool lane_tiny = t_prev || t_rc || t_rn || t_rnn || (read_un && own && tiny(un));
+        const bool lane_tiny = t_prev || t_rc || t_rn || t_rnn;
         if (__any(lane_tiny) || ((edge_rows >> r) & 1ull))
             row(ExactTag<true>{});
         else
@@ -556,7 +591,6 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         if (more) {
             rnn = nxt;
             t_rnn = tiny(nxt);
-            if (read_un) un = nun;
         }
     }
     fsum = facc;

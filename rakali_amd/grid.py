@@ -111,6 +111,20 @@ class Grid:
                                                      out.ctypes.data_as(ctypes.c_void_p), self.nx))
         return out
 
+    def set_field_device(self, name: str, ptr: int, ld: int) -> None:
+        """Device-to-device upload: `ptr` holds the (nx, ny) interior with x
+        fastest and `ld` >= nx elements per row (e.g. the data pointer of a
+        contiguous (ny, nx) torch tensor with ld = nx).  The caller has
+        finished writing it (synchronise its stream first)."""
+        check("rkg_upload_device", self._lib.rkg_upload_device(self.handle, FIELDS[name], ctypes.c_void_p(ptr),
+                                                               int(ld)))
+
+    def get_field_device(self, name: str, ptr: int, ld: int) -> None:
+        """Device-to-device download into `ptr` (layout as set_field_device);
+        returns once the copy is complete."""
+        check("rkg_download_device", self._lib.rkg_download_device(self.handle, FIELDS[name],
+                                                                   ctypes.c_void_p(ptr), int(ld)))
+
     def use_manning_field(self, enable: bool = True) -> None:
         """Per-cell Manning n (field "n", initialised to the scalar value)."""
         check("rkg_use_manning_field", self._lib.rkg_use_manning_field(self.handle, int(bool(enable))))

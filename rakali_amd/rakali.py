@@ -13,8 +13,9 @@ fallback, and a missing library raises ``FileNotFoundError``.
 Differences, all from features this build does not run: multilayer and
 non-hydrostatic simulations, nested edges and the semi-implicit scheme
 raise ``RuntimeError`` (the library returns -5);
-``save_restart`` writes the library's binary restart format instead of
-NetCDF; ``save_checkpoint`` writes classic NetCDF with scipy;
+``save_restart`` writes classic (64-bit offset) NetCDF in the reference's
+restart layout, not NetCDF-4; ``save_checkpoint`` writes classic NetCDF with
+scipy;
 ``to_xarray`` needs xarray.
 """
 from __future__ import annotations

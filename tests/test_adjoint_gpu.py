@@ -222,6 +222,32 @@ def test_torch_autograd_function():
     g.close()
 
 
+def test_torch_autograd_function_device_tensors():
+    """The same loss and gradients, bit for bit, from float64 tensors on the
+    GPU: the bed and roughness go to the grid device to device and the
+    gradients come back device to device, into tensors on the same device."""
+    torch = pytest.importorskip("torch")
+    from rakali_amd.adjoint import GaugeMisfit, bathymetry_roughness_loss, start_still_water
+    nx, ny, steps = 37, 21, 8
+    rng, b, n, _, _, _ = generic_state(nx, ny, 33)
+    g = gpu_grid(nx, ny, CFG, True, ("tidal", "wall", "open", "wall"), 0.0)
+    sess = GaugeMisfit(g, steps, 3, [(2, 3), (30, 8), (20, 17)], rng.normal(0.0, 0.05, (steps, 3)))
+    tb = torch.tensor(b, dtype=torch.float64, device="cuda", requires_grad=True)
+    tn = torch.tensor(n, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = bathymetry_roughness_loss(tb, tn, sess)
+    loss.backward()
+    assert loss.device.type == "cuda" and tb.grad.device.type == "cuda" and tn.grad.device.type == "cuda"
+    start_still_water(g, b, n)
+    l2 = sess.forward()
+    sess.backward()
+    sess.fold_still_water(0.0)
+    assert float(loss.detach().cpu()) == l2
+    assert np.array_equal(tb.grad.cpu().numpy(), sess.gradient("b"))
+    assert np.array_equal(tn.grad.cpu().numpy(), sess.gradient("n"))
+    sess.close()
+    g.close()
+
+
 def test_batched_sessions_equal_single_sessions_bitwise():
     """rkg_adjoint_forward_many / backward_many (sessions on their own
     streams, launches interleaved step by step) against each session run on
