@@ -271,6 +271,126 @@ __host__ __device__ __forceinline__ void face_rev(double g, const FState& L, dou
     if (bL > bR) lbL += lbmax; else lbR += lbmax;
 }
 
+// face_rev on the device for a face whose states and hydrostatic depths
+// are all wet and whose Riemann fan straddles the face: the branch the
+// forward run took there, with the forward quotients as shared-reciprocal
+// ones and the square roots as sqrt_pos (swe_core.hpp: the same bits as the
+// IEEE operations on the ranges checked here), the HLLC contact side from
+// the sign of its quotient, and the adjoint-only reciprocals 1/h from the
+// same refined reciprocals (the same bits as 1.0 / h).  Every operation
+// and accumulation is otherwise face_rev's, in its order, so the results
+// are face_rev's bits.  Returns false, having touched nothing, when a lane
+// that matters (`use`) breaks an assumption or holds a face value outside
+// the ranges of tiny(); the caller then runs face_rev.
+template <bool HLLC, bool YDIR>
+__device__ __forceinline__ bool face_rev_fast(double g, const FState& L, double bL, const FState& R, double bR,
+                                              const Face& lf, S3& lL, double& lbL, S3& lR, double& lbR, bool use) {
+    const double b_max = amax(bL, bR);
+    const double pl = L.h + bL - b_max, pr = R.h + bR - b_max;
+    const double hl = max0(pl), hr = max0(pr);
+    bool bad = !(L.h > DRY_TOLERANCE && R.h > DRY_TOLERANCE && hl > DRY_TOLERANCE && hr > DRY_TOLERANCE &&
+                 L.h <= 1.0e12 && R.h <= 1.0e12);
+    bad = bad || tiny(L.h) || tiny(L.hu) || tiny(L.hv) || tiny(bL) || tiny(R.h) || tiny(R.hu) || tiny(R.hv) ||
+          tiny(bR);
+    if (__any(use && bad)) return false;
+    const Recip rL = recip(L.h), rR = recip(R.h);
+    const double u_l = rdiv<false>(L.hu, rL), v_l = rdiv<false>(L.hv, rL);
+    const double u_r = rdiv<false>(R.hu, rR), v_r = rdiv<false>(R.hv, rR);
+    const double unl = YDIR ? v_l : u_l, utl = YDIR ? u_l : v_l;
+    const double unr = YDIR ? v_r : u_r, utr = YDIR ? u_r : v_r;
+    // riemann_rev's forward part (states (hl, nl, tl), (hr, nr, tr))
+    const double nl = hl * unl, tl = hl * utl, nr = hr * unr, tr = hr * utr;
+    const Recip rhl = recip(hl), rhr = recip(hr);
+    const double ul = rdiv<false>(nl, rhl), vl = rdiv<false>(tl, rhl);
+    const double ur = rdiv<false>(nr, rhr), vr = rdiv<false>(tr, rhr);
+    const double al = sqrt_pos(g * hl), ar = sqrt_pos(g * hr);
+    const double sl = amin(ul - al, ur - ar), sr = amax(ul + al, ur + ar);
+    const double span = sr - sl;
+    bad = !(sl < 0.0 && sr > 0.0 && span >= 1.0e-8 && span <= 1.0e12);
+    bool star_nonneg = true;
+    if (HLLC) {
+        const double dstar = hr * (ur - sr) - hl * (ul - sl);
+        const double num = sl * hr * (ur - sr) - sr * hl * (ul - sl);
+        bad = bad || !(fabs(dstar) > DRY_TOLERANCE && (num == 0.0 || fabs(num) >= 0x1p-900));
+        star_nonneg = num == 0.0 || ((num > 0.0) == (dstar > 0.0));
+    }
+    if (__any(use && bad)) return false;
+
+    // face_rev, before the Riemann solver
+    double lhl = 2.0 * hl * lf.c, lhr = 2.0 * hr * lf.a;
+    lR.h -= 2.0 * R.h * lf.a;
+    lL.h += 2.0 * L.h * lf.d;
+    const double lfm = lf.fh, lfn = YDIR ? lf.fhv : lf.fhu, lft = YDIR ? lf.fhu : lf.fhv;
+    // riemann_rev, Davis speeds and the HLL branch
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    {
+        const double flm = nl, fln = nl * ul + 0.5 * g * hl * hl, flt = nl * vl;
+        const double frm = nr, frn = nr * ur + 0.5 * g * hr * hr, frt = nr * vr;
+        double lflm = 0.0, lfln = 0.0, lflt = 0.0, lfrm = 0.0, lfrn = 0.0, lfrt = 0.0;
+        double lsl = 0.0, lsr = 0.0, lul = 0.0, lvl = 0.0, lur = 0.0, lvr = 0.0;
+        const double den = rdiv<false>(1.0, recip(span));
+        const double nm = sr * flm - sl * frm + sl * sr * (hr - hl);
+        const double fm = nm * den;
+        double lfm_tot = lfm, lden = 0.0;
+        if (HLLC) {
+            if (star_nonneg) { lfm_tot += lft * vl; lvl += lft * fm; }
+            else { lfm_tot += lft * vr; lvr += lft * fm; }
+        }
+        auto hll = [&](double lfq, double fl, double fr, double ql, double qr, double& lfl, double& lfr, double& lql,
+                       double& lqr) {
+            const double n = sr * fl - sl * fr + sl * sr * (qr - ql);
+            const double ln = lfq * den;
+            lden += lfq * n;
+            lsr += ln * (fl + sl * (qr - ql));
+            lsl += ln * (-fr + sr * (qr - ql));
+            lfl += ln * sr;
+            lfr -= ln * sl;
+            lqr += ln * sl * sr;
+            lql -= ln * sl * sr;
+        };
+        hll(lfm_tot, flm, frm, hl, hr, lflm, lfrm, a0, b0);
+        hll(lfn, fln, frn, nl, nr, lfln, lfrn, a1, b1);
+        if (!HLLC) hll(lft, flt, frt, tl, tr, lflt, lfrt, a2, b2);
+        const double ld = -lden * den * den;
+        lsr += ld;
+        lsl -= ld;
+        a1 += lflm + lfln * ul + lflt * vl;
+        lul += lfln * nl;
+        lvl += lflt * nl;
+        a0 += lfln * g * hl;
+        b1 += lfrm + lfrn * ur + lfrt * vr;
+        lur += lfrn * nr;
+        lvr += lfrt * nr;
+        b0 += lfrn * g * hr;
+        double lal = 0.0, lar = 0.0;
+        if ((ul - al) < (ur - ar)) { lul += lsl; lal -= lsl; } else { lur += lsl; lar -= lsl; }
+        if ((ul + al) > (ur + ar)) { lul += lsr; lal += lsr; } else { lur += lsr; lar += lsr; }
+        {
+            const double r = rdiv<false>(1.0, rhl);
+            a0 += lal * (0.5 * al) * r;
+            a1 += lul * r; a2 += lvl * r; a0 -= (lul * ul + lvl * vl) * r;
+        }
+        {
+            const double r = rdiv<false>(1.0, rhr);
+            b0 += lar * (0.5 * ar) * r;
+            b1 += lur * r; b2 += lvr * r; b0 -= (lur * ur + lvr * vr) * r;
+        }
+    }
+    // face_rev, after the Riemann solver
+    lhl += a0 + a1 * unl + a2 * utl;
+    lhr += b0 + b1 * unr + b2 * utr;
+    const double lunl = a1 * hl, lutl = a2 * hl, lunr = b1 * hr, lutr = b2 * hr;
+    const double lul = YDIR ? lutl : lunl, lvl = YDIR ? lunl : lutl;
+    const double lur = YDIR ? lutr : lunr, lvr = YDIR ? lunr : lutr;
+    { const double r = rdiv<false>(1.0, rL); lL.hu += lul * r; lL.hv += lvl * r; lL.h -= (lul * u_l + lvl * v_l) * r; }
+    { const double r = rdiv<false>(1.0, rR); lR.hu += lur * r; lR.hv += lvr * r; lR.h -= (lur * u_r + lvr * v_r) * r; }
+    double lbmax = 0.0;
+    if (pl > 0.0) { lL.h += lhl; lbL += lhl; lbmax -= lhl; }
+    if (pr > 0.0) { lR.h += lhr; lbR += lhr; lbmax -= lhr; }
+    if (bL > bR) lbL += lbmax; else lbR += lbmax;
+    return true;
+}
+
 // Adjoint of a face from the flux-divergence adjoints of the cells on its
 // low (ldl) and high (ldh) side: divergence (swe_core.hpp) transposed.
 // `inv` = 1/dx (x faces) or 1/dy (y faces).

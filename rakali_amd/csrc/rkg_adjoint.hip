@@ -267,6 +267,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
     // correctly rounded sine is a long dependent chain)
     const double eta = (j0 < 3 || j1 + 2 >= d.ny) ? tidal_eta(d, tau) : 0.0;
     const double g = d.g;
+    const bool fast = g >= 1.0e-3 && g <= 1.0e6;   // face_rev_fast's ranges assume an ordinary gravity
     const double rdy = 1.0 / d.dy;
     const long long P = d.plane;
 
@@ -327,7 +328,9 @@ __global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
         if (r >= -1 && r <= d.ny - 1) {
             const Face lf = adj::face_seed<true>(g, rdy, ld_r, ld_r1);
             double lbL = 0.0, lbR = 0.0;
-            adj::face_rev<HLLC, true>(g, hi_r, v1.b, lo_r1, v2.b, lf, lhi_r, lbL, llo_r1, lbR);
+            if (!(fast && adj::face_rev_fast<HLLC, true>(g, hi_r, v1.b, lo_r1, v2.b, lf, lhi_r, lbL, llo_r1, lbR,
+                                                          xin)))
+                adj::face_rev<HLLC, true>(g, hi_r, v1.b, lo_r1, v2.b, lf, lhi_r, lbL, llo_r1, lbR);
             acc[3].b += lbL;
             acc[4].b += lbR;
         }
@@ -388,6 +391,7 @@ __global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
     // only ghost columns need the tide
     const double eta = (c0 < 3 || c0 + 62 >= d.nx) ? tidal_eta(d, tau) : 0.0;
     const double g = d.g;
+    const bool fast = g >= 1.0e-3 && g <= 1.0e6;
     const double rdx = 1.0 / d.dx;
     const long long P = d.plane;
     for (int r = r0; r < r1; ++r) {
@@ -407,7 +411,8 @@ __global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
         double lbL = 0.0, lbR = 0.0;
         if (lane < WAVE - 1 && x >= -1 && x <= d.nx - 1) {   // the face between x and x + 1
             const Face lf = adj::face_seed<false>(g, rdx, ld_r, ld_next);
-            adj::face_rev<HLLC, false>(g, hi_x, v1.b, R, xp.b, lf, lhi_x, lbL, lR, lbR);
+            if (!(fast && adj::face_rev_fast<HLLC, false>(g, hi_x, v1.b, R, xp.b, lf, lhi_x, lbL, lR, lbR, true)))
+                adj::face_rev<HLLC, false>(g, hi_x, v1.b, R, xp.b, lf, lhi_x, lbL, lR, lbR);
         }
         adj::S3 llo_x{sh_up(lR.h), sh_up(lR.hu), sh_up(lR.hv)};
         const double lb_from_left = sh_up(lbR);
