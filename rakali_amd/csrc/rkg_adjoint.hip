@@ -117,7 +117,7 @@ __device__ __forceinline__ double block_sum256(double v, double* red4) {
 // adjoint of U*) is written.  STAGE 1: lam_out is the adjoint of U* and
 // lam_in (the adjoint of U^n) is added to.
 template <int STAGE>
-__global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
+__device__ __forceinline__ void adj_body(const AdjStage& a, int blk, int nblk) {
     const GridDesc& d = a.d;
     const double dt = a.hist[1];
     const SourceTerms src = adj_src(d, dt);
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
     double ldt = 0.0;
     double best = 0.0;   // stage 1: max CFL speed of U_s and its first cell
     long long bi = -1;
-    for (long long idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+    for (long long idx = blk * 256ll + threadIdx.x; idx < n; idx += (long long)nblk * 256) {
         const int j = (int)(idx / d.nx), i = (int)(idx % d.nx);
         const long long k = (long long)j * d.pitch + i;
         const double v0 = d.ah[k], v1 = d.ahu[k], v2 = d.ahv[k];
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
     }
     __shared__ double red4[4];
     const double tot = block_sum256(ldt, red4);
-    if (threadIdx.x == 0) a.part_dt[blockIdx.x] = tot;
+    if (threadIdx.x == 0) a.part_dt[blk] = tot;
     if (STAGE == 1 && a.pval) {
         __shared__ double sv[256];
         __shared__ long long si[256];
@@ -180,8 +180,13 @@ __global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) { a.pval[blockIdx.x] = sv[0]; a.pidx[blockIdx.x] = si[0]; }
+        if (threadIdx.x == 0) { a.pval[blk] = sv[0]; a.pidx[blk] = si[0]; }
     }
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_adj_body(AdjStage a) {
+    adj_body<STAGE>(a, blockIdx.x, gridDim.x);
 }
 
 struct Acc {   // adjoint accumulator of one cell: (h, hu, hv, b)
@@ -252,12 +257,12 @@ constexpr int NWIN = 6;   // accumulator rows r-3 .. r+2
 // folded into their source rows.  TY: a south or north edge is tidal, and
 // the adjoints of its ghost cells are kept in edge_lg for k_dt_rev.
 template <bool HLLC, int STAGE, bool TY>
-__global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
+__device__ __forceinline__ void adj_y(const AdjStage& a, int blk) {
     const GridDesc& d = a.d;
     const double tau = STAGE == 1 ? a.hist[0] : a.hist[0] + a.hist[1];   // bc%time of the stage
     const int lane = threadIdx.x;
     const int nstr = (d.nx + WAVE - 1) / WAVE;
-    const int sx = blockIdx.x % nstr, sy = blockIdx.x / nstr;
+    const int sx = blk % nstr, sy = blk / nstr;
     const int c0 = sx * WAVE;
     int j0, j1;
     block_range(d.ny, a.ty, sy, j0, j1);
@@ -365,6 +370,11 @@ __global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
     }
 }
 
+template <bool HLLC, int STAGE, bool TY>
+__global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
+    adj_y<HLLC, STAGE, TY>(a, blockIdx.x);
+}
+
 // Reverse of the x faces and x slopes of one stage (and the west / east
 // ghost columns) onto the stage input and the bed.  This direction is
 // row-local: every row is independent.  One wavefront per strip of at
@@ -375,12 +385,12 @@ __global__ __launch_bounds__(WAVE) void k_adj_y(AdjStage a) {
 // source columns by lane permutes.  TX: a west or east edge is tidal (see
 // k_adj_y's TY).
 template <bool HLLC, int STAGE, bool TX>
-__global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
+__device__ __forceinline__ void adj_x(const AdjStage& a, int blk) {
     const GridDesc& d = a.d;
     const double tau = STAGE == 1 ? a.hist[0] : a.hist[0] + a.hist[1];
     const int lane = threadIdx.x;
     const int nstr = (d.nx + AW - 1) / AW;
-    const int sx = blockIdx.x % nstr, sy = blockIdx.x / nstr;
+    const int sx = blk % nstr, sy = blk / nstr;
     int c0, c1;
     block_range(d.nx, AW, sx, c0, c1);
     const int r0 = sy * a.xrows, r1 = min(r0 + a.xrows, d.ny);
@@ -459,6 +469,11 @@ __global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
             a.grad_b[k] += mine.b;
         }
     }
+}
+
+template <bool HLLC, int STAGE, bool TX>
+__global__ __launch_bounds__(WAVE) void k_adj_x(AdjStage a) {
+    adj_x<HLLC, STAGE, TX>(a, blockIdx.x);
 }
 
 // d L / d eta_tidal from one tidal ghost cell (adjoints lg0 of h and lg1
@@ -565,15 +580,37 @@ __global__ __launch_bounds__(256) void k_gauge_loss(const double* __restrict__ h
 // walking all gauges would.  A cell gauge has one tap of weight 1; a
 // station the four of its bilinear weights ((1-wx)(1-wy), wx(1-wy),
 // (1-wx)wy, wx wy), since d eta / d h = d eta / d b = weight there.
-__global__ __launch_bounds__(256) void k_gauge_seed(const double* __restrict__ h, const double* __restrict__ bed,
-                                                   long long pitch, GaugeSpec G, const double* __restrict__ tgt,
-                                                   const int* __restrict__ cell_start,
-                                                   const int* __restrict__ cell_gauges,
-                                                   const double* __restrict__ cell_w,
-                                                   const long long* __restrict__ cell_pos, int ncells, double scale,
-                                                   double* __restrict__ lam_h, double* __restrict__ grad_b) {
-    const int u = blockIdx.x * 256 + threadIdx.x;
-    if (u >= ncells) return;
+struct SeedArgs {
+    const double* h;            // U^{n+1}
+    const double* bed;
+    long long pitch;
+    GaugeSpec G;
+    const double* tgt;          // the step's targets
+    const int* cell_start;
+    const int* cell_gauges;
+    const double* cell_w;
+    const long long* cell_pos;
+    int ncells;
+    double scale;
+    double* lam_h;
+    double* grad_b;
+};
+
+__device__ __forceinline__ void gauge_seed(const SeedArgs& A, int blk) {
+    const double* __restrict__ h = A.h;
+    const double* __restrict__ bed = A.bed;
+    const long long pitch = A.pitch;
+    const GaugeSpec& G = A.G;
+    const double* __restrict__ tgt = A.tgt;
+    const int* __restrict__ cell_start = A.cell_start;
+    const int* __restrict__ cell_gauges = A.cell_gauges;
+    const double* __restrict__ cell_w = A.cell_w;
+    const double scale = A.scale;
+    double* __restrict__ lam_h = A.lam_h;
+    double* __restrict__ grad_b = A.grad_b;
+    const int u = blk * 256 + threadIdx.x;
+    if (u >= A.ncells) return;
+    const long long* __restrict__ cell_pos = A.cell_pos;
     const long long o = cell_pos[u];
     double lh = lam_h[o], gb = grad_b[o];
     for (int q = cell_start[u]; q < cell_start[u + 1]; ++q) {
@@ -588,16 +625,42 @@ __global__ __launch_bounds__(256) void k_gauge_seed(const double* __restrict__ h
     grad_b[o] = gb;
 }
 
+__global__ __launch_bounds__(256) void k_gauge_seed(SeedArgs A) { gauge_seed(A, blockIdx.x); }
+
 
 
 // Scalar part of the reverse of one step (one workgroup).  scal[0] is the
 // adjoint of t_{s+1} carried from later steps, scal[1] that of a dt that
 // the next step reused.  Adds d L / d dt * d dt / d U at the cell of
 // maximum speed to lam (the adjoint of U_s) when dt was the CFL dt.
-__global__ __launch_bounds__(1024) void k_dt_rev(GridDesc d, const double* hist, const double* part_dt, int nbody,
-                                                const double* in2, const double* edge_lg, int edge_len,
-                                                const double* pval, const long long* pidx, int nargmax, double* lam,
-                                                double* scal) {
+struct DtArgs {
+    GridDesc d;                 // stage 1 (input U_s)
+    const double* hist;
+    const double* part_dt;      // 2 x nbody partial sums of d L / d dt
+    int nbody;
+    const double* in2;          // U* (the stage-2 input)
+    const double* edge_lg;
+    int edge_len;
+    const double* pval;
+    const long long* pidx;
+    int nargmax;
+    double* lam;                // the adjoint of U_s
+    double* scal;
+};
+
+__device__ __forceinline__ void dt_rev(const DtArgs& A) {
+    const GridDesc& d = A.d;
+    const double* hist = A.hist;
+    const double* part_dt = A.part_dt;
+    const int nbody = A.nbody;
+    const double* in2 = A.in2;
+    const double* edge_lg = A.edge_lg;
+    const int edge_len = A.edge_len;
+    const double* pval = A.pval;
+    const long long* pidx = A.pidx;
+    const int nargmax = A.nargmax;
+    double* lam = A.lam;
+    double* scal = A.scal;
     __shared__ double red[3][1024];
     __shared__ double sv[1024];
     __shared__ long long si[1024];
@@ -690,6 +753,68 @@ __global__ __launch_bounds__(1024) void k_dt_rev(GridDesc d, const double* hist,
     scal[0] = lam_t;
     scal[1] = carry;
 }
+
+__global__ __launch_bounds__(1024) void k_dt_rev(DtArgs A) { dt_rev(A); }
+
+// ---------------------------------------------------------------------
+// One batched reverse sweep over several sessions: each kernel below is one
+// launch over the blocks of every member, in the members' own block
+// decomposition (a table entry names the member and its block), so that a
+// member's arithmetic and sums are exactly those of its single-session
+// sweep.
+
+// A block of one member: the member, the block's index within the member's
+// launch and that launch's block count.
+struct BlockRef {
+    int member, blk, nblk, pad;
+};
+
+// A replay tile of one member (strip, segment, leader of its launch).
+struct RecTile {
+    int member, strip, seg, leader;
+};
+
+template <int STAGE, bool HLLC>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_replay_batch(
+    const GridDesc* __restrict__ descs, const RecTile* __restrict__ tiles, const int* __restrict__ parity) {
+    __shared__ TileLds L;
+    const RecTile t = tiles[blockIdx.x];
+    const GridDesc& d = descs[t.member];
+    stage_tile<STAGE, HLLC, true, true>(d, L, parity[t.member], MODE_STEP, d.t_end, [&](const GridDesc&) {
+        return make_int3(t.strip, t.seg, t.leader);
+    });
+}
+
+__global__ __launch_bounds__(256) void k_gauge_seed_batch(const SeedArgs* __restrict__ sa,
+                                                         const BlockRef* __restrict__ tab) {
+    const BlockRef r = tab[blockIdx.x];
+    gauge_seed(sa[r.member], r.blk);
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_adj_body_batch(const AdjStage* __restrict__ as,
+                                                       const BlockRef* __restrict__ tab) {
+    const BlockRef r = tab[blockIdx.x];
+    adj_body<STAGE>(as[r.member], r.blk, r.nblk);
+}
+
+// The tidal-edge variants (TY, TX) serve every member: they only add the
+// bookkeeping of tidal ghost cells, which other edges do not produce.
+template <bool HLLC, int STAGE>
+__global__ __launch_bounds__(WAVE) void k_adj_y_batch(const AdjStage* __restrict__ as,
+                                                     const BlockRef* __restrict__ tab) {
+    const BlockRef r = tab[blockIdx.x];
+    adj_y<HLLC, STAGE, true>(as[r.member], r.blk);
+}
+
+template <bool HLLC, int STAGE>
+__global__ __launch_bounds__(WAVE) void k_adj_x_batch(const AdjStage* __restrict__ as,
+                                                     const BlockRef* __restrict__ tab) {
+    const BlockRef r = tab[blockIdx.x];
+    adj_x<HLLC, STAGE, true>(as[r.member], r.blk);
+}
+
+__global__ __launch_bounds__(1024) void k_dt_rev_batch(const DtArgs* __restrict__ da) { dt_rev(da[blockIdx.x]); }
 
 // Still-water initial state: h = max(eta0 - b, 0) and its bed adjoint.
 __global__ __launch_bounds__(256) void k_fold_still_water(const double* __restrict__ bed, const double* __restrict__ lam_h,
@@ -1206,57 +1331,97 @@ void ensure_overlap(rkg_adjoint* a) {
     a->overlap_off = false;
 }
 
-// Reverse of step k of segment s (global step n0 + k).
-void bwd_reverse_step(rkg_adjoint* a, int s, int k, const BwdPlan& pl) {
+// Arguments of the kernels that reverse step k of the current segment
+// (global step n): the gauge seed, the two stages (STAGE 2 first), and the
+// dt/clock kernel.
+SeedArgs seed_args(rkg_adjoint* a, int k, int n, const BwdPlan& pl) {
     rkg_grid* g = a->g;
-    const size_t Pl = g->plane;
-    const int n = s * a->K + k;
+    SeedArgs A;
+    A.h = wsA(a, k + 1);
+    A.bed = g->buf + 6 * g->plane;
+    A.pitch = (long long)g->pitch;
+    A.G = gauge_spec(a);
+    A.tgt = a->tgt + (size_t)n * a->ng;
+    A.cell_start = a->cell_start;
+    A.cell_gauges = a->cell_gauges;
+    A.cell_w = a->cell_w;
+    A.cell_pos = a->cell_pos;
+    A.ncells = a->ncells;
+    A.scale = pl.scale;
+    A.lam_h = lamA(a);
+    A.grad_b = gradB(a);
+    return A;
+}
+
+AdjStage stage_args(rkg_adjoint* a, int k, int stage, const BwdPlan& pl) {
     const int nb = pl.nb;
-    double* part_dt = a->part;
-    double* pval = a->part + 2 * nb;
-    if (a->ng > 0)
-        hipLaunchKernelGGL(k_gauge_seed, dim3((a->ncells + 255) / 256), dim3(256), 0, g->stream, wsA(a, k + 1),
-                           g->buf + 6 * Pl, (long long)g->pitch, gauge_spec(a), a->tgt + (size_t)n * a->ng,
-                           a->cell_start, a->cell_gauges, a->cell_w, a->cell_pos, a->ncells, pl.scale, lamA(a),
-                           gradB(a));
     AdjStage st;
-    // stage 2: input U* (B_k), U^n = A_k
-    st.d = stage_desc(a, wsB(a, k), wsB(a, k), wsA(a, k + 1));
-    st.un = wsA(a, k);
-    st.div = wsD2(a, k);
-    st.lam_in = lamB(a);
-    st.lam_out = lamA(a);
     st.lam_div = lamDiv(a);
     st.grad_b = gradB(a);
     st.grad_n = gradN(a);
     st.hist = a->hist + HIST * k;
-    st.part_dt = part_dt + nb;
-    st.pval = nullptr;
-    st.pidx = nullptr;
-    st.edge_lg = a->edge_lg + pl.edge_size;
     st.edge_len = a->edge_len;
     st.ty = pl.tile_rows;
     st.xrows = pl.xrows;
-    hipLaunchKernelGGL(k_adj_body<2>, dim3(nb), dim3(256), 0, g->stream, st);
-    if (g->cfg.use_hllc) launch_faces<true, 2>(g, st);
-    else launch_faces<false, 2>(g, st);
-    // stage 1: input U^n (A_k)
-    st.d = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
-    st.un = nullptr;
-    st.div = wsD1(a, k);
-    st.lam_in = lamA(a);
-    st.lam_out = lamB(a);
-    st.part_dt = part_dt;
-    st.edge_lg = a->edge_lg;
-    st.pval = pl.cfl_dt ? pval : nullptr;
-    st.pidx = pl.cfl_dt ? a->part_idx : nullptr;
-    hipLaunchKernelGGL(k_adj_body<1>, dim3(nb), dim3(256), 0, g->stream, st);
-    if (g->cfg.use_hllc) launch_faces<true, 1>(g, st);
-    else launch_faces<false, 1>(g, st);
+    if (stage == 2) {   // input U* (B_k), U^n = A_k
+        st.d = stage_desc(a, wsB(a, k), wsB(a, k), wsA(a, k + 1));
+        st.un = wsA(a, k);
+        st.div = wsD2(a, k);
+        st.lam_in = lamB(a);
+        st.lam_out = lamA(a);
+        st.part_dt = a->part + nb;
+        st.pval = nullptr;
+        st.pidx = nullptr;
+        st.edge_lg = a->edge_lg + pl.edge_size;
+    } else {            // input U^n (A_k)
+        st.d = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
+        st.un = nullptr;
+        st.div = wsD1(a, k);
+        st.lam_in = lamA(a);
+        st.lam_out = lamB(a);
+        st.part_dt = a->part;
+        st.pval = pl.cfl_dt ? a->part + 2 * nb : nullptr;
+        st.pidx = pl.cfl_dt ? a->part_idx : nullptr;
+        st.edge_lg = a->edge_lg;
+    }
+    return st;
+}
+
+DtArgs dt_args(rkg_adjoint* a, int k, const AdjStage& st1, const BwdPlan& pl) {
+    DtArgs A;
+    A.d = st1.d;
+    A.hist = st1.hist;
+    A.part_dt = a->part;
+    A.nbody = pl.nb;
+    A.in2 = wsB(a, k);
+    A.edge_lg = a->edge_lg;
+    A.edge_len = a->edge_len;
+    A.pval = a->part + 2 * pl.nb;
+    A.pidx = a->part_idx;
+    A.nargmax = pl.nargmax;
+    A.lam = lamA(a);
+    A.scal = a->scal;
+    return A;
+}
+
+// Reverse of step k of segment s (global step n0 + k).
+void bwd_reverse_step(rkg_adjoint* a, int s, int k, const BwdPlan& pl) {
+    rkg_grid* g = a->g;
+    const int n = s * a->K + k;
+    const int nb = pl.nb;
+    if (a->ng > 0)
+        hipLaunchKernelGGL(k_gauge_seed, dim3((a->ncells + 255) / 256), dim3(256), 0, g->stream,
+                           seed_args(a, k, n, pl));
+    const AdjStage st2 = stage_args(a, k, 2, pl);
+    hipLaunchKernelGGL(k_adj_body<2>, dim3(nb), dim3(256), 0, g->stream, st2);
+    if (g->cfg.use_hllc) launch_faces<true, 2>(g, st2);
+    else launch_faces<false, 2>(g, st2);
+    const AdjStage st1 = stage_args(a, k, 1, pl);
+    hipLaunchKernelGGL(k_adj_body<1>, dim3(nb), dim3(256), 0, g->stream, st1);
+    if (g->cfg.use_hllc) launch_faces<true, 1>(g, st1);
+    else launch_faces<false, 1>(g, st1);
     // the step's dt and clock
-    hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(1024), 0, g->stream, st.d, st.hist, part_dt, nb,
-                       (const double*)wsB(a, k), (const double*)a->edge_lg, a->edge_len, pval, a->part_idx,
-                       pl.nargmax, lamA(a), a->scal);
+    hipLaunchKernelGGL(k_dt_rev, dim3(1), dim3(1024), 0, g->stream, dt_args(a, k, st1, pl));
 }
 
 // The sweep with the replay of segment s - 1 (into workspace set (s-1)&1,
@@ -1307,6 +1472,188 @@ int bwd_end(rkg_adjoint* a) {
 int bwd_finish(rkg_adjoint* a) {
     HIP_TRY(hipEventElapsedTime(&a->bwd_ms, a->ev[0], a->ev[1]));
     return RKG_OK;
+}
+
+// The reverse sweeps of n sessions as one sweep on the first session's
+// stream: per replay step two launches (both stages of every member) and
+// per reverse step eight (gauge seeds, the two stages' bodies and faces, the
+// dt/clock kernels), each over the blocks of every member.  Every kernel
+// argument of the whole sweep is known in advance (the sweep uses the first
+// workspace set of every session) and is uploaded once.  The members'
+// kernels and their block decompositions are those of their single-session
+// sweeps, so each member's gradients are the same bits.  All sessions must
+// use the same Riemann solver (HLL or HLLC).
+int bwd_batched(int n, rkg_adjoint* const* as, double loss_scale) {
+    const bool hllc = as[0]->g->cfg.use_hllc != 0;
+    const int nsteps = as[0]->nsteps, K = as[0]->K, nseg = as[0]->nseg;
+    std::vector<BwdPlan> pl(n);
+    int rc = RKG_OK;
+    for (int m = 0; m < n && !rc; ++m) rc = bwd_begin(as[m], loss_scale, 0, pl[m]);
+    if (rc) return rc;
+    for (int m = 0; m < n; ++m) use_set(as[m], 0);
+    hipStream_t S = as[0]->g->stream;
+
+    // block tables
+    std::vector<BlockRef> tb, ty, tx, tseed;
+    std::vector<RecTile> trec;
+    for (int m = 0; m < n; ++m) {
+        rkg_grid* g = as[m]->g;
+        const int nx = g->cfg.nx, ny = g->cfg.ny;
+        const int nb = pl[m].nb;
+        for (int b = 0; b < nb; ++b) tb.push_back(BlockRef{m, b, nb, 0});
+        const int yb = ((nx + WAVE - 1) / WAVE) * ((ny + pl[m].tile_rows - 1) / pl[m].tile_rows);
+        for (int b = 0; b < yb; ++b) ty.push_back(BlockRef{m, b, yb, 0});
+        const int xb = ((nx + AW - 1) / AW) * ((ny + pl[m].xrows - 1) / pl[m].xrows);
+        for (int b = 0; b < xb; ++b) tx.push_back(BlockRef{m, b, xb, 0});
+        if (as[m]->ng > 0) {
+            const int sb = (as[m]->ncells + 255) / 256;
+            for (int b = 0; b < sb; ++b) tseed.push_back(BlockRef{m, b, sb, 0});
+        }
+        const GridDesc d0 = make_desc(g);
+        const int ntiles = d0.nstrips * d0.nsegs;
+        for (int t = 0; t < ntiles; ++t) {   // the single-grid kernel's tile order and leader
+            RecTile r;
+            r.member = m;
+            r.strip = t % d0.nstrips;
+            r.seg = t / d0.nstrips;
+            r.leader = t == 0;
+            trec.push_back(r);
+        }
+    }
+    // per global step: replay descriptors (stage 1, 2), replay parities, and
+    // the reverse arguments (stage 2, stage 1, seeds, dt), n of each
+    const size_t szGD = sizeof(GridDesc), szAS = sizeof(AdjStage), szSA = sizeof(SeedArgs), szDA = sizeof(DtArgs);
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t oRec1 = 0, oRec2 = oRec1 + al(n * szGD), oPar = oRec2 + al(n * szGD), oSt2 = oPar + al(n * sizeof(int)),
+                 oSt1 = oSt2 + al(n * szAS), oSeed = oSt1 + al(n * szAS), oDt = oSeed + al(n * szSA),
+                 step_bytes = oDt + al(n * szDA);
+    const size_t tab_bytes = al(tb.size() * sizeof(BlockRef)) + al(ty.size() * sizeof(BlockRef)) +
+                             al(tx.size() * sizeof(BlockRef)) + al(std::max<size_t>(1, tseed.size()) * sizeof(BlockRef)) +
+                             al(trec.size() * sizeof(RecTile));
+    const size_t total = tab_bytes + step_bytes * (size_t)nsteps;
+    char* host = nullptr;
+    char* dev = nullptr;
+    hipEvent_t join = nullptr;
+    auto cleanup = [&]() {
+        if (host) (void)hipHostFree(host);
+        if (dev) (void)hipFree(dev);
+        if (join) (void)hipEventDestroy(join);
+    };
+    hipError_t e = hipHostMalloc((void**)&host, total, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&dev, total);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        cleanup();
+        set_error("batched adjoint sweep: cannot allocate its %zu-byte argument tables: %s", total, hipGetErrorString(e));
+        return RKG_ERR_HIP;
+    }
+    size_t off = 0;
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t o = off;
+        if (bytes) std::memcpy(host + o, src, bytes);
+        off += al(bytes);
+        return dev + o;
+    };
+    const BlockRef* dTb = (const BlockRef*)put(tb.data(), tb.size() * sizeof(BlockRef));
+    const BlockRef* dTy = (const BlockRef*)put(ty.data(), ty.size() * sizeof(BlockRef));
+    const BlockRef* dTx = (const BlockRef*)put(tx.data(), tx.size() * sizeof(BlockRef));
+    const BlockRef* dTs = (const BlockRef*)put(tseed.data(), tseed.size() * sizeof(BlockRef));
+    if (tseed.empty()) off += 256;
+    const RecTile* dTr = (const RecTile*)put(trec.data(), trec.size() * sizeof(RecTile));
+    off = tab_bytes;
+    for (int s = 0; s < nseg; ++s) {
+        for (int k = 0; k < seg_steps(as[0], s); ++k) {
+            char* h = host + tab_bytes + step_bytes * (size_t)(s * K + k);
+            for (int m = 0; m < n; ++m) {
+                rkg_adjoint* a = as[m];
+                GridDesc d1 = stage_desc(a, wsA(a, k), wsB(a, k), wsA(a, k + 1));
+                d1.rec = wsD1(a, k);
+                d1.hist = a->hist + HIST * k;
+                GridDesc d2 = d1;
+                d2.rec = wsD2(a, k);
+                d2.hist = nullptr;
+                std::memcpy(h + oRec1 + m * szGD, &d1, szGD);
+                std::memcpy(h + oRec2 + m * szGD, &d2, szGD);
+                const int par = a->ckpt_parity[s] ^ (k & 1);
+                std::memcpy(h + oPar + m * sizeof(int), &par, sizeof(int));
+                const AdjStage st2 = stage_args(a, k, 2, pl[m]);
+                const AdjStage st1 = stage_args(a, k, 1, pl[m]);
+                const SeedArgs sa = seed_args(a, k, s * K + k, pl[m]);
+                const DtArgs da = dt_args(a, k, st1, pl[m]);
+                std::memcpy(h + oSt2 + m * szAS, &st2, szAS);
+                std::memcpy(h + oSt1 + m * szAS, &st1, szAS);
+                std::memcpy(h + oSeed + m * szSA, &sa, szSA);
+                std::memcpy(h + oDt + m * szDA, &da, szDA);
+            }
+        }
+    }
+    // the sweep starts after every member's queued work (bwd_begin's clears)
+    for (int m = 1; m < n && e == hipSuccess; ++m) {
+        e = hipEventRecord(join, as[m]->g->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(S, join, 0);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, S);
+    if (e != hipSuccess) {
+        cleanup();
+        set_error("batched adjoint sweep: %s", hipGetErrorString(e));
+        return RKG_ERR_HIP;
+    }
+    const int nrec = (int)trec.size();
+    for (int s = nseg - 1; s >= 0 && !rc; --s) {
+        for (int m = 0; m < n && !rc; ++m) rc = bwd_replay_begin(as[m], s, pl[m], S);
+        if (rc) break;
+        const int Ks = seg_steps(as[0], s);
+        for (int k = 0; k < Ks; ++k) {
+            const char* d = dev + tab_bytes + step_bytes * (size_t)(s * K + k);
+            const GridDesc* r1 = (const GridDesc*)(d + oRec1);
+            const GridDesc* r2 = (const GridDesc*)(d + oRec2);
+            const int* par = (const int*)(d + oPar);
+            if (hllc) {
+                hipLaunchKernelGGL((k_replay_batch<1, true>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
+                hipLaunchKernelGGL((k_replay_batch<2, true>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
+            } else {
+                hipLaunchKernelGGL((k_replay_batch<1, false>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
+                hipLaunchKernelGGL((k_replay_batch<2, false>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
+            }
+        }
+        for (int k = Ks - 1; k >= 0; --k) {
+            const char* d = dev + tab_bytes + step_bytes * (size_t)(s * K + k);
+            const AdjStage* st2 = (const AdjStage*)(d + oSt2);
+            const AdjStage* st1 = (const AdjStage*)(d + oSt1);
+            if (!tseed.empty())
+                hipLaunchKernelGGL(k_gauge_seed_batch, dim3((int)tseed.size()), dim3(256), 0, S,
+                                   (const SeedArgs*)(d + oSeed), dTs);
+            hipLaunchKernelGGL(k_adj_body_batch<2>, dim3((int)tb.size()), dim3(256), 0, S, st2, dTb);
+            if (hllc) {
+                hipLaunchKernelGGL((k_adj_y_batch<true, 2>), dim3((int)ty.size()), dim3(WAVE), 0, S, st2, dTy);
+                hipLaunchKernelGGL((k_adj_x_batch<true, 2>), dim3((int)tx.size()), dim3(WAVE), 0, S, st2, dTx);
+            } else {
+                hipLaunchKernelGGL((k_adj_y_batch<false, 2>), dim3((int)ty.size()), dim3(WAVE), 0, S, st2, dTy);
+                hipLaunchKernelGGL((k_adj_x_batch<false, 2>), dim3((int)tx.size()), dim3(WAVE), 0, S, st2, dTx);
+            }
+            hipLaunchKernelGGL(k_adj_body_batch<1>, dim3((int)tb.size()), dim3(256), 0, S, st1, dTb);
+            if (hllc) {
+                hipLaunchKernelGGL((k_adj_y_batch<true, 1>), dim3((int)ty.size()), dim3(WAVE), 0, S, st1, dTy);
+                hipLaunchKernelGGL((k_adj_x_batch<true, 1>), dim3((int)tx.size()), dim3(WAVE), 0, S, st1, dTx);
+            } else {
+                hipLaunchKernelGGL((k_adj_y_batch<false, 1>), dim3((int)ty.size()), dim3(WAVE), 0, S, st1, dTy);
+                hipLaunchKernelGGL((k_adj_x_batch<false, 1>), dim3((int)tx.size()), dim3(WAVE), 0, S, st1, dTx);
+            }
+            hipLaunchKernelGGL(k_dt_rev_batch, dim3(n), dim3(1024), 0, S, (const DtArgs*)(d + oDt));
+        }
+    }
+    if (!rc) e = hipGetLastError();
+    // every member's stream continues after the sweep
+    if (!rc && e == hipSuccess) e = hipEventRecord(join, S);
+    for (int m = 1; m < n && !rc && e == hipSuccess; ++m) e = hipStreamWaitEvent(as[m]->g->stream, join, 0);
+    if (!rc && e != hipSuccess) { set_error("batched adjoint sweep: %s", hipGetErrorString(e)); rc = RKG_ERR_HIP; }
+    for (int m = 0; m < n && !rc; ++m) rc = bwd_end(as[m]);
+    // the tables are freed once every launch that reads them has finished
+    for (int m = 0; m < n; ++m) (void)hipStreamSynchronize(as[m]->g->stream);
+    cleanup();
+    if (rc) return rc;
+    for (int m = 0; m < n && !rc; ++m) rc = bwd_finish(as[m]);
+    return rc;
 }
 
 // Sessions of one batch: valid, distinct grids on distinct streams, with
@@ -1392,6 +1739,10 @@ int rkg_adjoint_forward_many(int n, rkg_adjoint* const* as, double* losses) {
 int rkg_adjoint_backward_many(int n, rkg_adjoint* const* as, double loss_scale) {
     int rc = check_batch(n, as);
     if (rc) return rc;
+    bool same_solver = true;
+    for (int m = 1; m < n; ++m) same_solver = same_solver && as[m]->g->cfg.use_hllc == as[0]->g->cfg.use_hllc;
+    if (same_solver) return bwd_batched(n, as, loss_scale);
+    // sessions mixing HLL and HLLC: their own kernels, interleaved on their streams
     std::vector<BwdPlan> pl(n);
     for (int m = 0; m < n && !rc; ++m) rc = bwd_begin(as[m], loss_scale, 0, pl[m]);
     for (int s = as[0]->nseg - 1; s >= 0 && !rc; --s) {
