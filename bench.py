@@ -469,10 +469,15 @@ def run_batched_adjoint(n_members: int, steps: int, checkpoint: int, fd_dirs: in
     # the size of the terms the directional derivative sums: a direction
     # along which they cancel makes the plain relative error measure the
     # difference quotient's rounding (about 1e-16 L / step) instead.
-    drng = np.random.default_rng(17)
+    # Each direction also gets its two one-sided quotients (L(+eps) - L(0)) /
+    # eps and (L(0) - L(-eps)) / eps.  Where they disagree by more than 1 %
+    # (plus the rounding of the loss) the segment [-eps, eps] crosses a kink
+    # of the piecewise-smooth scheme (a wet/dry switch, a minmod or max
+    # branch): the central difference there is not the derivative of either
+    # side, and such directions are counted apart.
     fd_res = {}
     for eps in (1e-6, 1e-8):
-        errs, scaled, worst = [], [], None
+        errs, scaled, worst, kinks, smooth = [], [], None, 0, []
         drng = np.random.default_rng(17)
         for _ in range(fd_dirs):
             db = [drng.normal(size=b.shape) for b in beds]
@@ -483,16 +488,29 @@ def run_batched_adjoint(n_members: int, steps: int, checkpoint: int, fd_dirs: in
             lm = forward_many(sess)
             for m in range(n_members):
                 fd = (lp[m] - lm[m]) / (2 * eps)
+                qp = (lp[m] - losses[m]) / eps
+                qm = (losses[m] - lm[m]) / eps
+                kink = abs(qp - qm) > 1e-2 * max(abs(qp), abs(qm)) + 1e-15 * abs(losses[m]) / eps
                 terms = np.concatenate([(grads[m][0] * db[m]).ravel(), (grads[m][1] * dn[m]).ravel()])
                 ad = float(np.sum(terms))
                 e = abs(fd - ad) / max(abs(fd), 1e-300)
                 errs.append(e)
+                kinks += int(kink)
+                if not kink:
+                    smooth.append(e)
                 scaled.append(abs(fd - ad) / float(np.sum(np.abs(terms))))
                 if worst is None or e > worst[0]:
-                    worst = (e, m, fd, ad, float(lp[m]))
-        fd_res[eps] = {"max_rel_err": max(errs), "median_rel_err": float(np.median(errs)),
-                       "max_err_over_sum_abs_terms": max(scaled),
-                       "worst": {"member": worst[1], "fd": worst[2], "adjoint": worst[3], "loss": worst[4]}}
+                    worst = (e, m, fd, ad, float(lp[m]), qp, qm, kink)
+        fd_res[eps] = {"max_rel_err": max(errs) if errs else None,
+                       "median_rel_err": float(np.median(errs)) if errs else None,
+                       "max_err_over_sum_abs_terms": max(scaled) if scaled else None,
+                       "kink_directions": kinks,
+                       "smooth_directions": len(smooth),
+                       "smooth_max_rel_err": max(smooth) if smooth else None,
+                       "smooth_median_rel_err": float(np.median(smooth)) if smooth else None,
+                       "worst": None if worst is None else
+                       {"member": worst[1], "fd": worst[2], "adjoint": worst[3], "loss": worst[4],
+                        "one_sided_plus": worst[5], "one_sided_minus": worst[6], "kink": bool(worst[7])}}
     for s in sess:
         s.close()
     for g in grids:

@@ -199,6 +199,13 @@ int rkg_sync(rkg_grid *g);
  * call returns the rows in use. */
 int rkg_set_tile_rows(rkg_grid *g, int rows);
 int rkg_get_tile_rows(rkg_grid *g, int *rows);
+/* Whether the stage kernels try their fast path (wavefronts whose faces
+ * and cells are all wet and subsonic skip the dry and shock branches and
+ * use cheaper exact sequences; any other wavefront takes the general code,
+ * so results are the same bits either way): 0 (default) when the launch
+ * has at least 6144 tiles, where it pays; 1 always; -1 never.  A tuning and
+ * testing knob with no counterpart in the reference. */
+int rkg_set_fast_path(rkg_grid *g, int mode);
 /* Keeps U^n of every step in three extra planes (24 B per cell per step of
  * extra traffic) so that RKG_FIELD_H0..HV0 can be read; off by default. */
 int rkg_set_rk_save(rkg_grid *g, int enable);

@@ -86,6 +86,7 @@ SIGNATURES = {
     "rkg_sync": (ctypes.c_int, [_P]),
     "rkg_set_tile_rows": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_get_tile_rows": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "rkg_set_fast_path": (ctypes.c_int, [_P, ctypes.c_int]),
     "rkg_last_error": (ctypes.c_char_p, []),
     "rkg_math_eval": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int]),
     "rkg_profile": (ctypes.c_int, [_P, ctypes.c_int]),

@@ -281,7 +281,7 @@ __host__ __device__ __forceinline__ void face_rev(double g, const FState& L, dou
 // and accumulation is otherwise face_rev's, in its order, so the results
 // are face_rev's bits.  Returns false, having touched nothing, when a lane
 // that matters (`use`) breaks an assumption or holds a face value outside
-// the ranges of tiny(); the caller then runs face_rev.
+// the ranges of unbounded(); the caller then runs face_rev.
 template <bool HLLC, bool YDIR>
 __device__ __forceinline__ bool face_rev_fast(double g, const FState& L, double bL, const FState& R, double bR,
                                               const Face& lf, S3& lL, double& lbL, S3& lR, double& lbR, bool use) {
@@ -290,8 +290,8 @@ __device__ __forceinline__ bool face_rev_fast(double g, const FState& L, double 
     const double hl = max0(pl), hr = max0(pr);
     bool bad = !(L.h > DRY_TOLERANCE && R.h > DRY_TOLERANCE && hl > DRY_TOLERANCE && hr > DRY_TOLERANCE &&
                  L.h <= 1.0e12 && R.h <= 1.0e12);
-    bad = bad || tiny(L.h) || tiny(L.hu) || tiny(L.hv) || tiny(bL) || tiny(R.h) || tiny(R.hu) || tiny(R.hv) ||
-          tiny(bR);
+    bad = bad || unbounded(L.h) || unbounded(L.hu) || unbounded(L.hv) || unbounded(bL) || unbounded(R.h) ||
+          unbounded(R.hu) || unbounded(R.hv) || unbounded(bR);
     if (__any(use && bad)) return false;
     const Recip rL = recip(L.h), rR = recip(R.h);
     const double u_l = rdiv<false>(L.hu, rL), v_l = rdiv<false>(L.hv, rL);

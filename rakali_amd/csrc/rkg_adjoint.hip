@@ -774,15 +774,18 @@ struct RecTile {
     int member, strip, seg, leader;
 };
 
-template <int STAGE, bool HLLC>
+struct RecTileOf {
+    RecTile t;
+    __device__ int3 operator()(const GridDesc&) const { return make_int3(t.strip, t.seg, t.leader); }
+};
+
+template <int STAGE, bool HLLC, bool FASTK>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_replay_batch(
     const GridDesc* __restrict__ descs, const RecTile* __restrict__ tiles, const int* __restrict__ parity) {
     __shared__ TileLds L;
     const RecTile t = tiles[blockIdx.x];
     const GridDesc& d = descs[t.member];
-    stage_tile<STAGE, HLLC, true, true>(d, L, parity[t.member], MODE_STEP, d.t_end, [&](const GridDesc&) {
-        return make_int3(t.strip, t.seg, t.leader);
-    });
+    stage_tile<STAGE, HLLC, true, true, RecTileOf, 0, FASTK>(d, L, parity[t.member], MODE_STEP, d.t_end, RecTileOf{t});
 }
 
 __global__ __launch_bounds__(256) void k_gauge_seed_batch(const SeedArgs* __restrict__ sa,
@@ -912,8 +915,12 @@ template <int STAGE>
 void launch_rec(rkg_grid* g, const GridDesc& d, int p, hipStream_t st) {
     const int ntiles = d.nstrips * d.nsegs;
     const double tlim = g->cfg.t_end;
-    if (g->cfg.use_hllc)
+    if (g->cfg.use_hllc && d.fast)
+        hipLaunchKernelGGL((k_stage<STAGE, true, true, true, 0, true>), dim3(ntiles), dim3(WAVE), 0, st, d, p, MODE_STEP, tlim);
+    else if (g->cfg.use_hllc)
         hipLaunchKernelGGL((k_stage<STAGE, true, true, true>), dim3(ntiles), dim3(WAVE), 0, st, d, p, MODE_STEP, tlim);
+    else if (d.fast)
+        hipLaunchKernelGGL((k_stage<STAGE, false, true, true, 0, true>), dim3(ntiles), dim3(WAVE), 0, st, d, p, MODE_STEP, tlim);
     else
         hipLaunchKernelGGL((k_stage<STAGE, false, true, true>), dim3(ntiles), dim3(WAVE), 0, st, d, p, MODE_STEP, tlim);
 }
@@ -1608,12 +1615,20 @@ int bwd_batched(int n, rkg_adjoint* const* as, double loss_scale) {
             const GridDesc* r1 = (const GridDesc*)(d + oRec1);
             const GridDesc* r2 = (const GridDesc*)(d + oRec2);
             const int* par = (const int*)(d + oPar);
-            if (hllc) {
-                hipLaunchKernelGGL((k_replay_batch<1, true>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
-                hipLaunchKernelGGL((k_replay_batch<2, true>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
+            // one launch over every member: the fast path by the total tile count
+            const bool fk = (long long)nrec >= FAST_MIN_TILES;
+            if (hllc && fk) {
+                hipLaunchKernelGGL((k_replay_batch<1, true, true>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
+                hipLaunchKernelGGL((k_replay_batch<2, true, true>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
+            } else if (hllc) {
+                hipLaunchKernelGGL((k_replay_batch<1, true, false>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
+                hipLaunchKernelGGL((k_replay_batch<2, true, false>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
+            } else if (fk) {
+                hipLaunchKernelGGL((k_replay_batch<1, false, true>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
+                hipLaunchKernelGGL((k_replay_batch<2, false, true>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
             } else {
-                hipLaunchKernelGGL((k_replay_batch<1, false>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
-                hipLaunchKernelGGL((k_replay_batch<2, false>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
+                hipLaunchKernelGGL((k_replay_batch<1, false, false>), dim3(nrec), dim3(WAVE), 0, S, r1, dTr, par);
+                hipLaunchKernelGGL((k_replay_batch<2, false, false>), dim3(nrec), dim3(WAVE), 0, S, r2, dTr, par);
             }
         }
         for (int k = Ks - 1; k >= 0; --k) {

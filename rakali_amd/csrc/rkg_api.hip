@@ -754,6 +754,13 @@ int rkg_get_stream(rkg_grid* g, void** stream) {
     return RKG_OK;
 }
 
+int rkg_set_fast_path(rkg_grid* g, int mode) {
+    if (!valid(g)) return RKG_ERR_BAD_HANDLE;
+    if (mode < -1 || mode > 1) { set_error("fast path mode must be -1 (never), 0 (by tile count) or 1 (always)"); return RKG_ERR_RANGE; }
+    g->fast_mode = mode;
+    return RKG_OK;
+}
+
 int rkg_get_tile_rows(rkg_grid* g, int* rows) {
     if (!valid(g)) return RKG_ERR_BAD_HANDLE;
     if (!rows) return RKG_ERR_RANGE;

@@ -35,9 +35,15 @@ struct BatchTile {
     int member, strip, seg, leader;
 };
 
+// The tile of a table entry, as stage_tile asks for it.
+struct TileOf {
+    BatchTile bt;
+    __device__ int3 operator()(const GridDesc&) const { return make_int3(bt.strip, bt.seg, bt.leader); }
+};
+
 // The tile loop of the persistent variant needs about 250 VGPRs to keep
 // the tile body free of spills, so it runs at two waves per SIMD.
-template <int STAGE, bool HLLC, bool PERSISTENT>
+template <int STAGE, bool HLLC, bool PERSISTENT, bool FASTK>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(PERSISTENT ? 2 : RKG_WAVES_PER_EU, 8))) void k_stage_batch(
     const GridDesc* __restrict__ descs, const BatchTile* __restrict__ tiles, int ntiles, int p,
     unsigned* __restrict__ queue) {
@@ -45,9 +51,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(PERSISTENT
     if (!PERSISTENT) {
         const int t = xcd_tile(blockIdx.x, ntiles);
         const BatchTile bt = tiles[t];
-        stage_tile<STAGE, HLLC, false, false>(descs[bt.member], L, p, MODE_STEP, descs[bt.member].t_end, [&](const GridDesc&) {
-            return make_int3(bt.strip, bt.seg, bt.leader);
-        });
+        stage_tile<STAGE, HLLC, false, false, TileOf, 0, FASTK>(descs[bt.member], L, p, MODE_STEP,
+                                                              descs[bt.member].t_end, TileOf{bt});
         return;
     }
     // The other stage's counter is idle during this launch: clear it for
@@ -60,9 +65,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(PERSISTENT
         const int t = __builtin_amdgcn_readfirstlane((int)slot);
         if (t >= ntiles) break;
         const BatchTile bt = tiles[t];
-        stage_tile<STAGE, HLLC, false, false>(descs[bt.member], L, p, MODE_STEP, descs[bt.member].t_end, [&](const GridDesc&) {
-            return make_int3(bt.strip, bt.seg, bt.leader);
-        });
+        stage_tile<STAGE, HLLC, false, false, TileOf, 0, FASTK>(descs[bt.member], L, p, MODE_STEP,
+                                                              descs[bt.member].t_end, TileOf{bt});
         __syncthreads();   // the next tile rewrites the LDS buffers and the slot
     }
 }
@@ -186,14 +190,21 @@ int refresh_speeds(rkg_ensemble* e) {
     return RKG_OK;
 }
 
-template <int STAGE, bool HLLC>
-void launch_one(rkg_ensemble* e, int p) {
+template <int STAGE, bool HLLC, bool FASTK>
+void launch_one_k(rkg_ensemble* e, int p) {
     if (e->schedule == 1)
-        hipLaunchKernelGGL((k_stage_batch<STAGE, HLLC, true>), dim3(e->persistent_blocks), dim3(WAVE), 0,
+        hipLaunchKernelGGL((k_stage_batch<STAGE, HLLC, true, FASTK>), dim3(e->persistent_blocks), dim3(WAVE), 0,
                            e->stream, e->descs, e->tiles, e->ntiles, p, e->queue);
     else
-        hipLaunchKernelGGL((k_stage_batch<STAGE, HLLC, false>), dim3(e->ntiles), dim3(WAVE), 0, e->stream,
+        hipLaunchKernelGGL((k_stage_batch<STAGE, HLLC, false, FASTK>), dim3(e->ntiles), dim3(WAVE), 0, e->stream,
                            e->descs, e->tiles, e->ntiles, p, e->queue);
+}
+
+// The kernels with the fast path when the launch holds enough tiles.
+template <int STAGE, bool HLLC>
+void launch_one(rkg_ensemble* e, int p) {
+    if ((long long)e->ntiles >= FAST_MIN_TILES) launch_one_k<STAGE, HLLC, true>(e, p);
+    else launch_one_k<STAGE, HLLC, false>(e, p);
 }
 
 void launch_stage(rkg_ensemble* e, int stage, int p) {

@@ -51,6 +51,7 @@ struct rkg_grid {
     double* mass_host;   // pinned [1]
     int parity;
     int ty;              // rows per stage tile, 0 = chosen from the grid size
+    int fast_mode;       // fast path: 0 by the launch's tile count, 1 always, -1 never
     bool speed_stale;
     bool prof;             // time each stage launch with events
     hipEvent_t ev[3];
@@ -179,6 +180,7 @@ inline GridDesc desc_of_config(const rkg_config& c, long long pitch, int ty) {
     d.ty = ty;
     d.nstrips = (c.nx + WAVE - 1) / WAVE;
     d.nsegs = (c.ny + d.ty - 1) / d.ty;
+    d.fast = (long long)d.nstrips * d.nsegs >= FAST_MIN_TILES ? 1 : 0;
     d.ssp_coeff = 1.0;
     return d;
 }
@@ -210,6 +212,7 @@ inline GridDesc make_desc(rkg_grid* g) {
         d.fdpy = g->frc + 5 * g->plane;
     }
     d.ssp_coeff = g->scheme == RKG_TIME_RK43 ? 2.0 : 1.0;  // rki_timestepper.F90:60-100
+    if (g->fast_mode != 0) d.fast = g->fast_mode > 0 ? 1 : 0;
     return d;
 }
 

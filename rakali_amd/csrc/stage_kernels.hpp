@@ -61,7 +61,7 @@ __device__ __forceinline__ bool tile_is_interior(const GridDesc& d, int strip, i
 // (strip, segment, leader); the leader is the single tile per grid and
 // launch that advances the control slot.  STAGE 1 and 2 are the fused
 // SSP-RK2 stages; STAGE 3 is one stage `gs` of a generic SSP scheme.
-template <int STAGE, bool HLLC, bool NF, bool REC, class Where, int FEAT = 0>
+template <int STAGE, bool HLLC, bool NF, bool REC, class Where, int FEAT = 0, bool FASTK = false>
 __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p, int mode, double tlim,
                                            Where where, const GenStage& gs = GenStage{}) {
     const int3 w = where(d);
@@ -148,10 +148,10 @@ __device__ __forceinline__ void stage_tile(const GridDesc& d, TileLds& L, int p,
     if (tile_is_interior(d, strip, seg)) {
         // no ghost cells: the tide (a correctly rounded sine, a long
         // dependent chain) is not needed
-        smax = process_tile<STAGE, HLLC, false, NF, REC, FEAT>(d, s, L, strip, seg, fsum, ssum);
+        smax = process_tile<STAGE, HLLC, false, NF, REC, FEAT, FASTK>(d, s, L, strip, seg, fsum, ssum);
     } else {
         s.eta_tidal = tidal_eta(d, t_tide);
-        smax = process_tile<STAGE, HLLC, true, NF, REC, FEAT>(d, s, L, strip, seg, fsum, ssum);
+        smax = process_tile<STAGE, HLLC, true, NF, REC, FEAT, FASTK>(d, s, L, strip, seg, fsum, ssum);
     }
     if (STAGE == 2 || STAGE == 3) {
         smax = wave_max(smax);
@@ -178,11 +178,12 @@ struct SingleGridTile {
     }
 };
 
-// FEAT: FEAT_FORCING / FEAT_SPONGE source terms.
-template <int STAGE, bool HLLC, bool NF, bool REC, int FEAT = 0>
+// FEAT: FEAT_FORCING / FEAT_SPONGE source terms.  FASTK: the fast path
+// compiled in (launches with d.fast set, FAST_MIN_TILES).
+template <int STAGE, bool HLLC, bool NF, bool REC, int FEAT = 0, bool FASTK = false>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(RKG_WAVES_PER_EU, 8))) void k_stage(GridDesc d, int p, int mode, double tlim) {
     __shared__ TileLds L;
-    stage_tile<STAGE, HLLC, NF, REC, SingleGridTile, FEAT>(d, L, p, mode, tlim, SingleGridTile{});
+    stage_tile<STAGE, HLLC, NF, REC, SingleGridTile, FEAT, FASTK>(d, L, p, mode, tlim, SingleGridTile{});
 }
 
 // One stage of a generic SSP scheme, single grid.

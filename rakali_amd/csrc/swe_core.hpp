@@ -87,11 +87,14 @@ __device__ __forceinline__ double rdiv(double n, const Recip& r) {
     return fma(-t, r.y, q0);
 }
 
-// Non-zero and outside [2^-300, 2^40] in magnitude, or not finite: a value
-// for which some quotient of the step could leave the range where
-// rdiv<false> is exact (the bound 2^40 also keeps every square root and
-// reciprocal argument of the fast row path finite and in range).
-__device__ __forceinline__ bool tiny(double x) {
+// Non-zero and below 2^-300 in magnitude: a value for which some quotient
+// of the step could leave the range where rdiv<false> is exact.
+__device__ __forceinline__ bool tiny(double x) { return fabs(x) < 0x1p-300 && x != 0.0; }
+
+// Non-zero and outside [2^-300, 2^40] in magnitude, or not finite: besides
+// tiny values, the values for which the fast path's range arguments (below)
+// do not hold; the kernels that carry the fast path test their rows with it.
+__device__ __forceinline__ bool unbounded(double x) {
     const double a = fabs(x);
     return x != 0.0 && !(a >= 0x1p-300 && a <= 0x1p40);
 }
@@ -115,6 +118,9 @@ __device__ __forceinline__ double sqrt_pos(double x) {
 }
 __device__ __forceinline__ bool tiny(const Cell& c) {
     return tiny(c.h) || tiny(c.hu) || tiny(c.hv) || tiny(c.b);
+}
+__device__ __forceinline__ bool unbounded(const Cell& c) {
+    return unbounded(c.h) || unbounded(c.hu) || unbounded(c.hv) || unbounded(c.b);
 }
 
 // minmod, rki_kernel_flux.F90:380-393
@@ -287,13 +293,13 @@ __device__ __forceinline__ Face face_flux(double g, const FState& L, double bL, 
 //  * every face state and hydrostatic depth exceeds DRY_TOLERANCE: the
 //    velocity quotients need no dry branch, the wave-speed arguments g h
 //    lie above 2^-767 and are finite (the row's inputs are bounded by
-//    tiny()), so sqrt_pos gives sqrt's bits, and the speeds are Davis's;
+//    unbounded()), so sqrt_pos gives sqrt's bits, and the speeds are Davis's;
 //  * s_l < 0 < s_r (the HLL star region) with s_r - s_l in [1e-8, 1e12]:
 //    1/(s_r - s_l) is the shared-reciprocal quotient of 1.0, the same bits
 //    as the division;
 //  * HLLC: |denom_star| > DRY_TOLERANCE and num = 0 or |num| >= 2^-900.
 //    Only the sign of s_star = num / denom_star is used; |denom_star| stays
-//    below 2^83 (depths below 2^41 for inputs bounded by tiny(), and
+//    below 2^83 (depths below 2^41 for inputs bounded by unbounded(), and
 //    |u - s| <= 2 (s_r - s_l) <= 2e12), so the quotient does not underflow
 //    to zero and
 //    s_star >= 0 exactly when num is zero or num and denom_star have the

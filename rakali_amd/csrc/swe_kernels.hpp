@@ -17,6 +17,11 @@ namespace rk {
 
 constexpr int WAVE = 64;        // wavefront width on CDNA
 constexpr int MAX_TC = 10;      // MAX_TIDAL_CONSTITUENTS
+// Tiles a stage launch needs before the fast path (swe_core.hpp) pays: its
+// wave-uniform tests and fallbacks cost latency, which a launch of a few
+// waves per SIMD cannot hide (at 2048 tiles, 1024^2 cells, the stage
+// kernels are 5 % slower with it; at 16384, 4096^2, 5 % faster).
+constexpr long long FAST_MIN_TILES = 6144;
 
 // Time-control slot.  Two slots ping-pong: a step reads slot p (state at
 // the start of the step) and its stage-1 kernel writes slot 1-p.
@@ -68,6 +73,7 @@ struct GridDesc {
     double dt_fixed, t_end;
     int cfl_interval;
     int ty;                       // rows per tile segment (<= 64)
+    int fast;                     // the launch may take the fast path (FAST_MIN_TILES)
     int nstrips, nsegs;
     CtrlSlot* ctrl;               // [2]
 };
@@ -383,7 +389,7 @@ __device__ __forceinline__ void sponge_cell(const GridDesc& d, double dt, int c,
     }
 }
 
-template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC, int FEAT = 0>
+template <int STAGE, bool HLLC, bool EDGE, bool NF, bool REC, int FEAT = 0, bool FASTK = false>
 __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx& s, TileLds& L,
                                                int strip, int seg, double& fsum, double& ssum) {
     constexpr bool FRC = (FEAT & FEAT_FORCING) != 0;
@@ -396,8 +402,11 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
     const int nrows = j1 - j0;
     const double g = d.g;
     const long long pitch = d.pitch;
-    // the fast row path's ranges assume a gravity of ordinary size
-    const bool try_fast = g >= 1.0e-3 && g <= 1.0e6;
+    // the fast path's ranges assume a gravity of ordinary size
+    const bool try_fast = FASTK && g >= 1.0e-3 && g <= 1.0e6;
+    // the value test of the rows: kernels with the fast path also keep its
+    // range arguments (unbounded()), the others the division's (tiny())
+    auto odd = [](const Cell& x) { return FASTK ? unbounded(x) : tiny(x); };
     double smax = 0.0;
     double facc = 0.0;  // sum of the lane's flux_h, rows in order
     double sacc = 0.0;  // sum of the lane's sponge change of h
@@ -417,7 +426,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         const Cell m1 = ld<EDGE, FEAT>(d, s, c0 + 63, j);
         const Cell p0 = ld<EDGE, FEAT>(d, s, c0 + 64, j);
         L.hr[0][lane] = p0.h; L.hr[1][lane] = p0.hu; L.hr[2][lane] = p0.hv; L.hr[3][lane] = p0.b;
-        edge_tiny = tiny(a) || tiny(bl) || tiny(cc) || tiny(m1) || tiny(p0);
+        edge_tiny = odd(a) || odd(bl) || odd(cc) || odd(m1) || odd(p0);
         if (c0 + WAVE - 1 < d.nx) {
             const Cell m2 = ld<EDGE, FEAT>(d, s, c0 + 62, j);
             const Cell p1 = ld<EDGE, FEAT>(d, s, c0 + 65, j);
@@ -448,26 +457,37 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         FState bc;
         cell_states(bm, rc, rn, bc, tc);
         fyb = face_flux<HLLC, true, true>(g, tm, bm.b, bc, rc.b);
-        t_prev = tiny(bm);
-        t_rc = tiny(rc);
-        t_rn = tiny(rn);
-        t_rnn = tiny(rnn);
+        t_prev = odd(bm);
+        t_rc = odd(rc);
+        t_rn = odd(rn);
+        t_rnn = odd(rnn);
     }
     const bool own = c < d.nx;
+    Cell un;   // U^n at (c, j) for the RK2 average
+    un.h = 0.0; un.hu = 0.0; un.hv = 0.0; un.b = 0.0;
+    if (!FASTK && read_un && own) {
+        const long long k = (long long)j0 * pitch + c;
+        un.h = d.ah[k]; un.hu = d.ahu[k]; un.hv = d.ahv[k];
+    }
     __syncthreads();
 
     for (int j = j0; j < j1; ++j) {
         const int r = j - j0;
         const bool more = j + 1 < j1;
-        Cell nxt;
+        Cell nxt, nun;
         if (more) nxt = ld<EDGE, FEAT>(d, s, c, j + 3);
-        // U^n at (c, j) for the RK2 average, needed only at the end of the
-        // row (its load hides behind the faces)
-        Cell un;
-        un.h = 0.0; un.hu = 0.0; un.hv = 0.0; un.b = 0.0;
-        if (read_un && own) {
-            const long long k = (long long)j * pitch + c;
-            un.h = d.ah[k]; un.hu = d.ahu[k]; un.hv = d.ahv[k];
+        // U^n at (c, j) for the RK2 average, needed at the end of the row.
+        // Kernels with the fast path load it here (its load hides behind the
+        // faces, and it saves the registers of a second row); the others
+        // fetch it a row ahead, which a launch of few waves needs.
+        if constexpr (FASTK) {
+            if (read_un && own) {
+                const long long k = (long long)j * pitch + c;
+                un.h = d.ah[k]; un.hu = d.ahu[k]; un.hv = d.ahv[k];
+            }
+        } else if (more && read_un && own) {
+            const long long k = (long long)(j + 1) * pitch + c;
+            nun.h = d.ah[k]; nun.hu = d.ahu[k]; nun.hv = d.ahv[k];
         }
         FState tn;
         Face fyt;
@@ -476,7 +496,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
             // faces and cells take the fast path (swe_core.hpp) where the
             // wavefront allows it; lanes up to the first column past the
             // grid feed an owned cell through their x-face
-            const bool fast = !EX && try_fast;
+            const bool fast = FASTK && !EX && try_fast;
             const bool use_x = c <= d.nx;
             // y: slope of row j+1 and the face between rows j and j+1
             FState bn;
@@ -531,7 +551,7 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
                 if (STAGE == 1 && SPG) sponge_cell(d, s.dt, c, j, h, hu, hv, sacc);
                 // U^n with a tiny value in any lane: the rest of the cell
                 // update with plain divisions
-                const bool ex_un = EX || (read_un && __any(tiny(un)));
+                const bool ex_un = EX || (read_un && __any(odd(un)));
                 if (STAGE == 2) {
                     // rki_kernel_fused.F90:435-463
                     h = 0.5 * (un.h + h);
@@ -590,7 +610,8 @@ __device__ __forceinline__ double process_tile(const GridDesc& d, const StageCtx
         t_rn = t_rnn;
         if (more) {
             rnn = nxt;
-            t_rnn = tiny(nxt);
+            t_rnn = odd(nxt);
+            if (!FASTK && read_un) un = nun;
         }
     }
     fsum = facc;

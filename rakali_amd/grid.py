@@ -304,3 +304,8 @@ class Grid:
     @tile_rows.setter
     def tile_rows(self, rows: int) -> None:
         check("rkg_set_tile_rows", self._lib.rkg_set_tile_rows(self.handle, int(rows)))
+
+    def set_fast_path(self, mode: int) -> None:
+        """Stage kernels' fast path: 0 by the launch's tile count (default),
+        1 always, -1 never; the results are the same bits."""
+        check("rkg_set_fast_path", self._lib.rkg_set_fast_path(self.handle, int(mode)))
