@@ -8,15 +8,16 @@
  * last-ulp rounding differs between platforms: glibc's pow(x, 1/3) is not
  * correctly rounded for about 7e-4 of inputs and its sin for about 1.5e-3.
  * These functions evaluate in double-double arithmetic from IEEE basic
- * operations and fma only, so they are deterministic on every IEEE machine
- * and round correctly except in cases closer to a rounding midpoint than
- * about 1e-26 relative.
+ * operations and fma only (x**(1/3) also binary32 ones for its seed and its
+ * ln x term), so they are deterministic on every IEEE machine and round
+ * correctly except in cases closer to a rounding midpoint than about 1e-26
+ * relative (4e-24 for x**(1/3)).
  *
  * Methods (own implementation):
- *   inverse cube root seeded by exponent splitting and a quadratic fit,
- *   four division-free Newton steps, one double-double residual
- *   correction; the exponent 1.0/3.0 is the double 1/3 - 2^-54/3, applied
- *   as the factor exp(-(2^-54/3) ln x).
+ *   inverse cube root seeded by exponent splitting and a float fit, two
+ *   division-free Newton steps, one double-double residual correction;
+ *   the exponent 1.0/3.0 is the double 1/3 - 2^-54/3, applied as the
+ *   factor 1 - (2^-54/3) ln x, ln x from a float polynomial.
  *   sin/cos by Cody-Waite reduction with pi/2 in three doubles and Taylor
  *   series (through x^25 / x^26) in double-double.
  * TEST INFRASTRUCTURE: part of the oracle.
@@ -67,50 +68,60 @@ static inline crm_dd crm_dd_mul(crm_dd x, crm_dd y)
     return crm_fast_two_sum(p, e);
 }
 
-/* ln x to ~1e-11 relative, x > 0 normal (atanh series). */
-static inline double crm_log_approx(double x)
-{
-    uint64_t u = crm_d2bits(x);
-    int e = (int)((u >> 52) & 0x7ff) - 1023;
-    double m = crm_bits2d((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull); /* [1,2) */
-    if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
-    double s = (m - 1.0) / (m + 1.0);
-    double s2 = s * s;
-    double poly = 1.0 / 11.0;
-    poly = 1.0 / 9.0 + s2 * poly;
-    poly = 1.0 / 7.0 + s2 * poly;
-    poly = 1.0 / 5.0 + s2 * poly;
-    poly = 1.0 / 3.0 + s2 * poly;
-    poly = 1.0 + s2 * poly;
-    return (double)e * 0.6931471805599453 + 2.0 * s * poly;
-}
-
-/* x**(1.0/3.0) for normal x > 0, rounded to nearest. */
+/* x**(1.0/3.0) for 2^-900 <= x < 2^1000, rounded to nearest.
+ *   x = m 2^e, m in [1, 2), e = 3q + r, y = m 2^r in [1, 8);
+ *   y^(-1/3): degree-4 float fit of m^(-1/3) on [1, 2] (relative error
+ *   < 4.6e-5) times 2^(-r/3), two division-free Newton steps in double;
+ *   r0 = y rr^2 2^q, then one Newton step on the residual x - r0^3 in
+ *   double-double;
+ *   exponent 1/3 - 2^-54/3: times 1 - (2^-54/3) ln x, ln x = e ln 2 + ln m,
+ *   ln m = f Q(f), f = m - 1, Q a degree-7 float polynomial (absolute
+ *   error < 2e-7, moving the result by < 4e-24 relative). */
 static inline double crm_pow13(double x)
 {
     uint64_t u = crm_d2bits(x);
-    int e = (int)((u >> 52) & 0x7ff) - 1023;
-    double m = crm_bits2d((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull); /* [1,2) */
-    int q = (e >= 0) ? e / 3 : -((-e + 2) / 3); /* floor(e / 3) */
-    int r = e - 3 * q;                          /* 0, 1, 2 */
-    double y = m * (double)(1 << r);            /* [1, 8), exact */
-    /* y^(-1/3): quadratic fit of m^(-1/3) on [1, 2] (relative error
-     * < 3.3e-3) times 2^(-r/3), then four division-free Newton steps */
-    double rr = 1.37911983 + m * (-0.47289307 + m * 0.09056511);
-    rr = rr * (r == 0 ? 1.0 : (r == 1 ? 0.7937005259840998 : 0.6299605249474366));
-    for (int it = 0; it < 4; ++it) rr = rr + (rr * (1.0 - y * ((rr * rr) * rr))) * (1.0 / 3.0);
-    double r0 = ((y * rr) * rr) * crm_bits2d((uint64_t)(q + 1023) << 52); /* ~x^(1/3) */
+    uint32_t eb = (uint32_t)(u >> 52);   /* biased exponent, x > 0 */
+    uint32_t w = eb + 2046u;             /* 3 (q + 1023) + r */
+    uint32_t qq = w / 3u;                /* q + 1023 */
+    uint32_t r = w - 3u * qq;            /* 0, 1, 2 */
+    uint64_t mant = u & 0x000FFFFFFFFFFFFFull;
+    double m = crm_bits2d(mant | 0x3FF0000000000000ull);                 /* [1, 2) */
+    double y = crm_bits2d(mant | ((uint64_t)(1023u + r) << 52));         /* [1, 8) */
+    float mf = (float)m;
+    float s = 0x1.c73184p-6f;
+    s = fmaf(s, mf, -0x1.bb1becp-3f);
+    s = fmaf(s, mf, 0x1.5e8accp-1f);
+    s = fmaf(s, mf, -0x1.2992e0p+0f);
+    s = fmaf(s, mf, 0x1.aa913ep+0f);
+    s = s * (r == 0 ? 1.0f : (r == 1 ? 0x1.965feap-1f : 0x1.428a30p-1f)); /* 2^(-r/3) */
+    double rr = (double)s;
+    for (int it = 0; it < 2; ++it) {
+        double e = fma(-y, (rr * rr) * rr, 1.0);
+        rr = fma(rr, e * (1.0 / 3.0), rr);
+    }
+    double r0 = ((y * rr) * rr) * crm_bits2d((uint64_t)qq << 52); /* ~x^(1/3) */
     /* residual x - r0^3 in double-double, Newton correction res/(3 r0^2) */
     double p = r0 * r0;
     double pl = fma(r0, r0, -p);
     double t3 = p * r0;
     double t3l = fma(p, r0, -t3);
-    double res = (x - t3) - (t3l + pl * r0);
-    double inv = ((rr * rr) * (1.0 / 3.0)) * crm_bits2d((uint64_t)(1023 - 2 * q) << 52);
+    double res = (x - t3) - fma(pl, r0, t3l);
+    double inv = ((rr * rr) * (1.0 / 3.0)) * crm_bits2d((uint64_t)(3069u - 2u * qq) << 52);
     double corr = res * inv;
-    /* exponent is 1/3 - 2^-54/3: multiply by exp(-(2^-54/3) ln x) */
-    double t = -(crm_log_approx(x) * (1.0 / 3.0)) * 5.551115123125783e-17;
-    return r0 + (corr + r0 * t);
+    /* ln x, then the factor 1 - (2^-54/3) ln x */
+    float f = (float)(m - 1.0);
+    float lq = -0x1.124e52p-7f;
+    lq = fmaf(lq, f, 0x1.644d52p-5f);
+    lq = fmaf(lq, f, -0x1.b5a860p-4f);
+    lq = fmaf(lq, f, 0x1.6a3d52p-3f);
+    lq = fmaf(lq, f, -0x1.f53e4ep-3f);
+    lq = fmaf(lq, f, 0x1.54b45ep-2f);
+    lq = fmaf(lq, f, -0x1.fff898p-2f);
+    lq = fmaf(lq, f, 0x1.fffff8p-1f);
+    float lm = lq * f;
+    double lnx = fma((double)((int)eb - 1023), 0x1.62e42fefa39efp-1, (double)lm);
+    double t = lnx * -0x1.5555555555555p-56; /* -(2^-54 / 3) ln x */
+    return r0 + fma(r0, t, corr);
 }
 
 /* 1/n! as double-double, n = 0..26 */

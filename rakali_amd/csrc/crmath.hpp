@@ -8,14 +8,17 @@
 // rounding is platform specific (ROCm's ocml pow and glibc's pow disagree
 // for a large fraction of inputs; glibc itself misrounds pow(x, 1/3) for
 // ~7e-4 of inputs).  These routines use only IEEE +, -, *, /, fma and
-// exact bit manipulation, in double-double where it matters, so they give
-// the same bits on every IEEE machine and the correctly rounded result
-// except within ~1e-26 relative of a rounding midpoint.
+// exact bit manipulation (pow13 also IEEE binary32 steps for its seed and
+// its ln x term), in double-double where it matters, so
+// they give the same bits on every IEEE machine and the correctly rounded
+// result except within ~1e-26 relative of a rounding midpoint (~4e-24 for
+// pow13).
 //
-// pow13: inverse cube root seeded by exponent splitting and a quadratic
-// fit, four division-free Newton steps, one double-double residual
-// correction, then the factor exp(-(2^-54/3) ln x) because the double
-// 1.0/3.0 is 1/3 - 2^-54/3.
+// pow13: inverse cube root seeded by exponent splitting and a float fit,
+// two division-free Newton steps, one double-double residual correction,
+// then the factor exp(-(2^-54/3) ln x) because the double 1.0/3.0 is
+// 1/3 - 2^-54/3 (ln x from a float polynomial; its error moves the result
+// by < 4e-24 relative).
 // sin/cos: Cody-Waite reduction by pi/2 held in three doubles, Taylor
 // series to x^25 / x^26 in double-double.
 #pragma once
@@ -62,45 +65,62 @@ __host__ __device__ __forceinline__ dd mul(dd x, dd y) {
     return fast_two_sum(p, e);
 }
 
-// ln x to ~1e-11 relative for normal x > 0 (atanh series).
-__host__ __device__ __forceinline__ double log_approx(double x) {
-    const uint64_t u = d2bits(x);
-    int e = (int)((u >> 52) & 0x7ff) - 1023;
-    double m = bits2d((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
-    if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
-    const double s = (m - 1.0) / (m + 1.0);
-    const double s2 = s * s;
-    double poly = 1.0 / 11.0;
-    poly = 1.0 / 9.0 + s2 * poly;
-    poly = 1.0 / 7.0 + s2 * poly;
-    poly = 1.0 / 5.0 + s2 * poly;
-    poly = 1.0 / 3.0 + s2 * poly;
-    poly = 1.0 + s2 * poly;
-    return (double)e * 0.6931471805599453 + 2.0 * s * poly;
-}
-
-// x**(1.0/3.0) for normal x > 0.
+// x**(1.0/3.0) for 2^-900 <= x < 2^1000 (below, the residual x - r0^3
+// falls among subnormals and the last bit can be wrong).
+//   x = m 2^e, m in [1, 2), e = 3q + r, y = m 2^r in [1, 8);
+//   y^(-1/3): a degree-4 float fit of m^(-1/3) on [1, 2] (relative error
+//   < 4.6e-5) times 2^(-r/3), then two division-free Newton steps in
+//   double (error < 1e-16 before rounding);
+//   r0 = y rr^2 2^q ~ x^(1/3), then one Newton step on the residual
+//   x - r0^3 taken in double-double (error ~1e-31);
+//   the double 1.0/3.0 is 1/3 - 2^-54/3: the factor exp(-(2^-54/3) ln x)
+//   is applied as 1 - (2^-54/3) ln x, with ln x = e ln 2 + ln m and ln m
+//   from a degree-8 float polynomial f Q(f), f = m - 1 (absolute error
+//   < 2e-7, which moves the result by < 4e-24 relative).
+// Float steps are IEEE binary32 operations (conversions round to nearest),
+// so the sequence is as deterministic as the double steps.
 __host__ __device__ __forceinline__ double pow13(double x) {
     const uint64_t u = d2bits(x);
-    const int e = (int)((u >> 52) & 0x7ff) - 1023;
-    const double m = bits2d((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
-    const int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);  // floor(e / 3)
-    const int r = e - 3 * q;
-    const double y = m * (double)(1 << r);  // [1, 8)
-    // y^(-1/3): quadratic seed (relative error < 3.3e-3), four Newton steps
-    double rr = 1.37911983 + m * (-0.47289307 + m * 0.09056511);
-    rr = rr * (r == 0 ? 1.0 : (r == 1 ? 0.7937005259840998 : 0.6299605249474366));
-    for (int it = 0; it < 4; ++it) rr = rr + (rr * (1.0 - y * ((rr * rr) * rr))) * (1.0 / 3.0);
-    const double r0 = ((y * rr) * rr) * bits2d((uint64_t)(q + 1023) << 52);
+    const uint32_t eb = (uint32_t)(u >> 52);   // biased exponent, x > 0
+    const uint32_t w = eb + 2046u;             // 3 (q + 1023) + r
+    const uint32_t qq = w / 3u;                // q + 1023
+    const uint32_t r = w - 3u * qq;            // 0, 1, 2
+    const uint64_t mant = u & 0x000FFFFFFFFFFFFFull;
+    const double m = bits2d(mant | 0x3FF0000000000000ull);
+    const double y = bits2d(mant | ((uint64_t)(1023u + r) << 52));
+    const float mf = (float)m;
+    float s = 0x1.c73184p-6f;
+    s = fmaf(s, mf, -0x1.bb1becp-3f);
+    s = fmaf(s, mf, 0x1.5e8accp-1f);
+    s = fmaf(s, mf, -0x1.2992e0p+0f);
+    s = fmaf(s, mf, 0x1.aa913ep+0f);
+    s = s * (r == 0 ? 1.0f : (r == 1 ? 0x1.965feap-1f : 0x1.428a30p-1f));   // 2^(-r/3)
+    double rr = (double)s;
+    for (int it = 0; it < 2; ++it) {
+        const double e = fma(-y, (rr * rr) * rr, 1.0);
+        rr = fma(rr, e * (1.0 / 3.0), rr);
+    }
+    const double r0 = ((y * rr) * rr) * bits2d((uint64_t)qq << 52);
     const double p = r0 * r0;
     const double pl = fma(r0, r0, -p);
     const double t3 = p * r0;
     const double t3l = fma(p, r0, -t3);
-    const double res = (x - t3) - (t3l + pl * r0);
-    const double inv = ((rr * rr) * (1.0 / 3.0)) * bits2d((uint64_t)(1023 - 2 * q) << 52);
+    const double res = (x - t3) - fma(pl, r0, t3l);
+    const double inv = ((rr * rr) * (1.0 / 3.0)) * bits2d((uint64_t)(3069u - 2u * qq) << 52);   // x^(-2/3) / 3
     const double corr = res * inv;
-    const double t = -(log_approx(x) * (1.0 / 3.0)) * 5.551115123125783e-17;  // 2^-54
-    return r0 + (corr + r0 * t);
+    const float f = (float)(m - 1.0);
+    float lq = -0x1.124e52p-7f;
+    lq = fmaf(lq, f, 0x1.644d52p-5f);
+    lq = fmaf(lq, f, -0x1.b5a860p-4f);
+    lq = fmaf(lq, f, 0x1.6a3d52p-3f);
+    lq = fmaf(lq, f, -0x1.f53e4ep-3f);
+    lq = fmaf(lq, f, 0x1.54b45ep-2f);
+    lq = fmaf(lq, f, -0x1.fff898p-2f);
+    lq = fmaf(lq, f, 0x1.fffff8p-1f);
+    const float lm = lq * f;
+    const double lnx = fma((double)((int)eb - 1023), 0x1.62e42fefa39efp-1, (double)lm);
+    const double t = lnx * -0x1.5555555555555p-56;   // -(2^-54 / 3) ln x
+    return r0 + fma(r0, t, corr);
 }
 
 // 1/n! as double-double, n = 0..26.

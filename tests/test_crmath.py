@@ -49,6 +49,12 @@ def test_pow13_correctly_rounded():
     rnd = random.Random(3)
     xs = [math.exp(rnd.uniform(math.log(1e-7), math.log(1e4))) for _ in range(6000)]
     xs += [8.0, 27.0, 1.0, 0.001, 1e-6, 64.0, 2.0 ** -30]
+    # the documented domain [2^-900, 2^1000): every exponent residue mod 3,
+    # mantissas at both ends of [1, 2) and at the cube roots of 2 and 4
+    xs += [math.exp(rnd.uniform(-900 * math.log(2.0), 999 * math.log(2.0))) for _ in range(1500)]
+    for e in range(-900, 1000, 37):
+        for m in (1.0, 1.0 + 2.0 ** -52, 1.2599210498948732, 1.5874010519681994, 2.0 - 2.0 ** -52):
+            xs.append(math.ldexp(m, e))
     bad = [x for x in xs if ev(0, x) != float(Decimal(x) ** c)]
     assert not bad, bad[:5]
 
