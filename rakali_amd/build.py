@@ -92,6 +92,16 @@ def _run(cmd: list[str], out: Path, verbose: bool) -> Path:
     return out
 
 
+def unit_flags(src: Path) -> list[str]:
+    """Extra flags of one translation unit: the stage kernels (stage_feat*)
+    use LLVM's ILP-first machine scheduler, which is about 1 % faster on the
+    4096x4096 step at the same occupancy (three waves per SIMD, no spills);
+    it changes the instruction order, not the arithmetic."""
+    if src.stem.startswith("stage_feat"):
+        return ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+    return []
+
+
 def _compile_objects(sources: list[Path], outdir: Path, verbose: bool) -> list[str]:
     """Compiles each translation unit to an object, the units in parallel
     (each one instantiates many stage kernels and takes tens of seconds)."""
@@ -99,7 +109,7 @@ def _compile_objects(sources: list[Path], outdir: Path, verbose: bool) -> list[s
 
     def one(src: Path) -> str:
         obj = outdir / (src.stem + ".o")
-        cmd = [hipcc(), *flags, "-c", str(src), "-o", str(obj)]
+        cmd = [hipcc(), *flags, *unit_flags(src), "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         res = subprocess.run(cmd, capture_output=True, text=True)

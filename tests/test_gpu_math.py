@@ -11,14 +11,18 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("fn,lo,hi,log", [(0, 1e-6, 1e4, True), (1, -200.0, 200.0, False),
-                                         (2, -200.0, 200.0, False)])
+@pytest.mark.parametrize("fn,lo,hi,log", [(0, 1e-6, 1e4, True), (0, 2.0 ** -900, 2.0 ** 999, True),
+                                         (1, -200.0, 200.0, False), (2, -200.0, 200.0, False)])
 def test_device_math_matches_host_and_oracle(fn, lo, hi, log):
     from rakali_amd import _lib
     L = _lib.load()
     rng = np.random.default_rng(11 + fn)
     n = 1_000_000
     x = np.exp(rng.uniform(np.log(lo), np.log(hi), n)) if log else rng.uniform(lo, hi, n)
+    if fn == 0:   # every exponent residue mod 3 with mantissas at both ends of [1, 2)
+        e = np.arange(max(-900, int(np.floor(np.log2(lo)))), min(999, int(np.floor(np.log2(hi)))))
+        m = np.array([1.0, 1.0 + 2.0 ** -52, 1.2599210498948732, 1.5874010519681994, 2.0 - 2.0 ** -52])
+        x[: e.size * m.size] = np.ldexp(np.repeat(m[None, :], e.size, 0), e[:, None]).ravel()
     x = np.ascontiguousarray(x)
     dev = np.empty(n)
     host = np.empty(n)
