@@ -314,7 +314,9 @@ int rkg_copy_bandwidth(long long bytes, int reps, double *gbs, double *ms_per_co
  * (on_device = 0) or the current GPU: fn 0 = x**(1.0/3.0) (Manning,
  * rki_kernel_fused.F90:268), 1 = sin, 2 = cos (tidal surface
  * rki_boundary.F90:96-100, Coriolis rki_kernel_fused.F90:212-213).  Both
- * sides return identical, correctly rounded bits. */
+ * sides return identical bits, correctly rounded except for inputs whose
+ * exact result lies within about 1e-26 (x**(1/3), for x in [2^-900,
+ * 2^1000): 4e-24) relative of a rounding midpoint. */
 int rkg_math_eval(int fn, const double *x, double *y, int n, int on_device);
 
 /* ------------------------------------------------------------------------
