@@ -76,8 +76,8 @@ def main():
     faulthandler.enable()
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build's path is "
-              "single-GPU; multi-GPU sharding is described in DESIGN.md, not measured.",
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the step is measured "
+              "on one GPU's grid; sharding over several GPUs is designed but not built or measured.",
               file=sys.stderr)
         return 2
 
